@@ -1,0 +1,267 @@
+"""ctypes binding of the C ABI in ``include/ccsd_amd.h``.
+
+The product path is the HIP library ``lib/libccsd_amd.so``; there is no CPU
+fallback.  :func:`load` raises if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libccsd_amd.so")
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+ERRORS = {
+    0: "CCSD_OK", -1: "CCSD_ERR_ARG", -2: "CCSD_ERR_HIP", -3: "CCSD_ERR_BUDGET", -4: "CCSD_ERR_NONFINITE",
+    -5: "CCSD_ERR_NOT_CONVERGED", -6: "CCSD_ERR_DENOMINATOR", -7: "CCSD_ERR_NO_DEVICE", -8: "CCSD_ERR_STATE",
+}
+
+EXPORTED = (
+    "ccsd_default_options", "ccsd_create", "ccsd_create_synthetic", "ccsd_solve", "ccsd_iterate", "ccsd_reset",
+    "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
+    "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
+    "ccsd_selftest_gemm",
+)
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("switch_threshold", C.c_double), ("final_fp64_sweeps", C.c_int),
+        ("diis_dim", C.c_int), ("diis_start", C.c_int), ("recompute_v4", C.c_int), ("verbose", C.c_int),
+        ("device", C.c_int), ("hbm_budget_bytes", C.c_size_t),
+    ]
+
+
+class Problem(C.Structure):
+    _fields_ = [("n_occ", C.c_int), ("n_virt", C.c_int)] + [
+        (n, C.POINTER(C.c_double)) for n in ("fock", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("e_corr", C.c_double), ("e_mp2", C.c_double), ("last_delta_e", C.c_double), ("max_delta_t2", C.c_double),
+        ("iterations", C.c_int), ("converged", C.c_int), ("diis_fallbacks", C.c_int), ("lowp_iterations", C.c_int),
+        ("fp64_iterations", C.c_int), ("seconds_total", C.c_double), ("seconds_per_iter", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GradInfo(C.Structure):
+    _fields_ = [("lambda_iterations", C.c_int), ("converged", C.c_int), ("lambda_residual", C.c_double),
+                ("seconds", C.c_double)]
+
+
+class CCSDError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: {ERRORS.get(code, code)}")
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.ccsd_default_options.argtypes = [C.POINTER(Options)]
+    lib.ccsd_create.argtypes = [C.POINTER(Problem), C.POINTER(Options), C.POINTER(vp)]
+    lib.ccsd_create_synthetic.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double, C.POINTER(Options), C.POINTER(vp)]
+    lib.ccsd_solve.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Result)]
+    lib.ccsd_iterate.argtypes = [vp, C.c_int, C.c_int, dp]
+    lib.ccsd_reset.argtypes = [vp]
+    lib.ccsd_residual.argtypes = [vp, dp, dp, dp, dp]
+    lib.ccsd_energy.argtypes = [vp, dp, dp, dp]
+    lib.ccsd_get_amplitudes.argtypes = [vp, dp, dp]
+    lib.ccsd_set_amplitudes.argtypes = [vp, dp, dp]
+    lib.ccsd_energy_trace.argtypes = [vp, dp, C.c_int]
+    lib.ccsd_backward.argtypes = [vp] + [dp] * 7 + [C.POINTER(GradInfo)]
+    lib.ccsd_plan_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.ccsd_plan_bytes.restype = C.c_size_t
+    lib.ccsd_last_timings.argtypes = [vp, dp, C.c_int]
+    lib.ccsd_destroy.argtypes = [vp]
+    lib.ccsd_destroy.restype = None
+    lib.ccsd_strerror.argtypes = [C.c_int]
+    lib.ccsd_strerror.restype = C.c_char_p
+    lib.ccsd_debug_get.argtypes = [vp, C.c_char_p, dp, C.c_long]
+    lib.ccsd_debug_get.restype = C.c_long
+    lib.ccsd_selftest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_double,
+                                       dp, C.c_int]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CCSDError(rc, what)
+
+
+def default_options(**kw):
+    lib = load()
+    o = Options()
+    lib.ccsd_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        setattr(o, k, v)
+    return o
+
+
+def p2(n):
+    return n * (n - 1) // 2
+
+
+class Context:
+    """One engine context (device memory owned by the library)."""
+
+    def __init__(self, handle, O, V):
+        self.h = handle
+        self.O, self.V = O, V
+        self.po, self.pv, self.ov = p2(O), p2(V), O * V
+        self.lib = load()
+
+    @classmethod
+    def from_problem(cls, prob, **opts):
+        lib = load()
+        O, V = prob["O"], prob["V"]
+        keep = {k: np.ascontiguousarray(prob[k], dtype=np.float64) for k in
+                ("f", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")}
+        p = Problem(O, V, _ptr(keep["f"]), _ptr(keep["oooo"]), _ptr(keep["ooov"]), _ptr(keep["oovv"]),
+                    _ptr(keep["ovov"]), _ptr(keep["ovvv"]), _ptr(keep["vvvv"]))
+        o = default_options(**opts)
+        h = C.c_void_p()
+        _check(lib.ccsd_create(C.byref(p), C.byref(o), C.byref(h)), "ccsd_create")
+        return cls(h, O, V)
+
+    @classmethod
+    def synthetic(cls, O, V, seed, scale, **opts):
+        lib = load()
+        o = default_options(**opts)
+        h = C.c_void_p()
+        _check(lib.ccsd_create_synthetic(O, V, seed, scale, C.byref(o), C.byref(h)), "ccsd_create_synthetic")
+        return cls(h, O, V)
+
+    def close(self):
+        if self.h:
+            self.lib.ccsd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, tol=1e-10, max_iter=1000, allow_unconverged=False):
+        r = Result()
+        rc = self.lib.ccsd_solve(self.h, tol, max_iter, C.byref(r))
+        if rc != 0 and not (allow_unconverged and rc == -5):
+            raise CCSDError(rc, "ccsd_solve")
+        return r.as_dict()
+
+    def iterate(self, n, precision="fp64"):
+        e = np.zeros(max(n, 1))
+        _check(self.lib.ccsd_iterate(self.h, n, PRECISIONS.get(precision, precision), _ptr(e)), "ccsd_iterate")
+        return e[:n]
+
+    def reset(self):
+        _check(self.lib.ccsd_reset(self.h), "ccsd_reset")
+
+    def residual(self, t1, t2p):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64)
+        r1 = np.zeros(self.ov)
+        r2 = np.zeros(self.po * self.pv)
+        _check(self.lib.ccsd_residual(self.h, _ptr(t1), _ptr(t2p), _ptr(r1), _ptr(r2)), "ccsd_residual")
+        return r1.reshape(self.O, self.V), r2.reshape(self.po, self.pv)
+
+    def energy(self, t1, t2p):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64)
+        e = C.c_double()
+        _check(self.lib.ccsd_energy(self.h, _ptr(t1), _ptr(t2p), C.byref(e)), "ccsd_energy")
+        return e.value
+
+    def get_amplitudes(self):
+        t1 = np.zeros(self.ov)
+        t2 = np.zeros(self.po * self.pv)
+        _check(self.lib.ccsd_get_amplitudes(self.h, _ptr(t1), _ptr(t2)), "ccsd_get_amplitudes")
+        return t1.reshape(self.O, self.V), t2.reshape(self.po, self.pv)
+
+    def set_amplitudes(self, t1, t2p):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64)
+        _check(self.lib.ccsd_set_amplitudes(self.h, _ptr(t1), _ptr(t2p)), "ccsd_set_amplitudes")
+
+    def energy_trace(self):
+        n = self.lib.ccsd_energy_trace(self.h, None, 0)
+        out = np.zeros(max(n, 1))
+        self.lib.ccsd_energy_trace(self.h, _ptr(out), n)
+        return out[:n]
+
+    def timings(self):
+        ms = np.zeros(5)
+        self.lib.ccsd_last_timings(self.h, _ptr(ms), 5)
+        return dict(zip(("pp_ladder", "ring", "wmbej_quad", "wmbij", "other"), ms.tolist()))
+
+    def debug(self, name):
+        n = self.lib.ccsd_debug_get(self.h, name.encode(), None, 0)
+        if n < 0:
+            raise CCSDError(int(n), "ccsd_debug_get")
+        out = np.zeros(n)
+        rc = self.lib.ccsd_debug_get(self.h, name.encode(), _ptr(out), n)
+        if rc < 0:
+            raise CCSDError(int(rc), "ccsd_debug_get")
+        return out
+
+    def backward(self):
+        O, V = self.O, self.V
+        N = O + V
+        g = dict(f=np.zeros((N, N)), oooo=np.zeros((self.po, self.po)), ooov=np.zeros((self.po, self.ov)),
+                 oovv=np.zeros((self.po, self.pv)), ovov=np.zeros((self.ov, self.ov)),
+                 ovvv=np.zeros((self.ov, self.pv)), vvvv=np.zeros((self.pv, self.pv)))
+        info = GradInfo()
+        _check(self.lib.ccsd_backward(self.h, *[_ptr(g[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv",
+                                                                      "vvvv")], C.byref(info)), "ccsd_backward")
+        return g, {k: getattr(info, k) for k, _ in info._fields_}
+
+
+def plan_bytes(O, V, precision="fp64"):
+    return load().ccsd_plan_bytes(O, V, PRECISIONS.get(precision, precision))
+
+
+def strerror(code):
+    return load().ccsd_strerror(code).decode()
+
+
+def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision="fp64"):
+    """C = alpha op(A) op(B) + beta C through the device GEMM engine."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    Cm = np.ascontiguousarray(C_in, dtype=np.float64).copy()
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    _check(load().ccsd_selftest_gemm(M, N, K, int(ta), int(tb), alpha, _ptr(A), _ptr(B), beta, _ptr(Cm),
+                                     PRECISIONS.get(precision, precision)), "ccsd_selftest_gemm")
+    return Cm

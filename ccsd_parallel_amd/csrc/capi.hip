@@ -1,0 +1,590 @@
+// C ABI and the amplitude-iteration driver.
+//
+// Driver semantics follow CCSD_Memory_optimized::CCSD (ccsd_memory_optimized.cpp:16-567):
+// T1 = 0 and T2 = <ab||ij>/D (:473-490), energy of that guess (:501-502), then
+// repeated Jacobi updates T <- R(T)/D (:513-557) with |E_new - E_old| < toler
+// as the stopping test.  Additions: optional FP64 DIIS on (T1, packed T2)
+// following the Pulay convention of hartree_fock_solver.cpp:1252-1326, the
+// mixed-precision schedule, and error codes instead of silent stops.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "adjoint.hpp"
+#include "engine.hpp"
+#include "lowprec.hpp"
+
+struct ccsd_ctx {
+  ccsd::Ctx c;
+};
+
+namespace ccsd {
+namespace {
+
+#define TRY(x)                      \
+  do {                              \
+    int _rc = (x);                  \
+    if (_rc != CCSD_OK) return _rc; \
+  } while (0)
+
+#define HIPTRY(x)                            \
+  do {                                       \
+    hipError_t _e = (x);                     \
+    if (_e != hipSuccess) {                  \
+      c->last_error = hipGetErrorString(_e); \
+      return CCSD_ERR_HIP;                   \
+    }                                        \
+  } while (0)
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Orbital energies of the synthetic generator (same definition as the device blocks).
+std::vector<double> synthetic_fock(int O, int V, uint64_t seed) {
+  const uint64_t s = seed ^ 0xD1B54A32D192ED03ull;
+  std::vector<double> eo(O), ev(V);
+  for (int p = 0; p < O + V; ++p) {
+    uint64_t h = mix64(s + ((uint64_t)p + 1ull) * 0x9E3779B97F4A7C15ull);
+    double u = (double)(h >> 11) * 1.1102230246251565e-16;
+    if (p < O) eo[p] = -1.5 + 1.2 * u;
+    else ev[p - O] = 0.2 + 2.8 * u;
+  }
+  std::sort(eo.begin(), eo.end());
+  std::sort(ev.begin(), ev.end());
+  const int N = O + V;
+  std::vector<double> f((size_t)N * N, 0.0);
+  for (int i = 0; i < O; ++i) f[(size_t)i * N + i] = eo[i];
+  for (int a = 0; a < V; ++a) f[(size_t)(O + a) * N + O + a] = ev[a];
+  return f;
+}
+
+int check_fock(Ctx* c) {
+  const long N = c->d.O + c->d.V;
+  for (double v : c->h_f)
+    if (!std::isfinite(v)) return CCSD_ERR_NONFINITE;
+  for (long i = 0; i < c->d.O; ++i)
+    for (long a = c->d.O; a < N; ++a)
+      if (std::fabs(c->h_f[i * N + i] - c->h_f[a * N + a]) < 1e-10) {
+        c->last_error = "vanishing denominator f_ii - f_aa";
+        return CCSD_ERR_DENOMINATOR;
+      }
+  return CCSD_OK;
+}
+
+int init_device(Ctx* c, const ccsd_options* o, int O, int V) {
+  if (O < 2 || V < 2) return CCSD_ERR_ARG;
+  ccsd_default_options(&c->opt);
+  if (o) c->opt = *o;
+  if (c->opt.diis_dim < 0 || c->opt.diis_dim > 64 || c->opt.final_fp64_sweeps < 0) return CCSD_ERR_ARG;
+  if (c->opt.precision < CCSD_PREC_FP64 || c->opt.precision > CCSD_PREC_BF16X3) return CCSD_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCSD_ERR_NO_DEVICE;
+  if (c->opt.device < 0 || c->opt.device >= ndev) return CCSD_ERR_ARG;
+  c->device = c->opt.device;
+  HIPTRY(hipSetDevice(c->device));
+  HIPTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->d = make_dims(O, V);
+  return plan_and_allocate(c, c->opt.precision);
+}
+
+void release(Ctx* c) {
+  if (c->device >= 0) (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  adjoint_free(c);
+  lowprec_free(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+int mp2_start(Ctx* c) {
+  k_mp2_guess(c->amp, c->oovv, c->f, c->d, c->stream);
+  HIPTRY(hipGetLastError());
+  return CCSD_OK;
+}
+
+void diis_reset(Ctx* c) {
+  c->diis_count = 0;
+  c->diis_next = 0;
+  std::fill(c->B.begin(), c->B.end(), 0.0);
+}
+
+// Bordered Pulay system with diagonal scaling; false if singular or wild.
+bool diis_coefficients(const std::vector<double>& Bfull, int m, std::vector<double>& coef) {
+  const int n = m + 1;
+  std::vector<double> A((size_t)n * n, -1.0), b(n, 0.0), s(n, 1.0);
+  double dmax = 0.0;
+  for (int i = 0; i < m; ++i) dmax = std::max(dmax, std::fabs(Bfull[(size_t)i * m + i]));
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return false;
+  for (int i = 0; i < m; ++i) {
+    double dii = std::fabs(Bfull[(size_t)i * m + i]);
+    s[i] = 1.0 / std::sqrt(dii > 0.0 ? dii : 1.0);
+  }
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) A[(size_t)i * n + j] = Bfull[(size_t)i * m + j] * s[i] * s[j];
+  for (int i = 0; i < m; ++i) { A[(size_t)i * n + m] = -s[i]; A[(size_t)m * n + i] = -s[i]; }
+  A[(size_t)m * n + m] = 0.0;
+  b[m] = -1.0;
+  // Gaussian elimination with partial pivoting
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int r = k + 1; r < n; ++r)
+      if (std::fabs(A[(size_t)r * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = r;
+    if (std::fabs(A[(size_t)piv * n + k]) < 1e-300) return false;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int r = k + 1; r < n; ++r) {
+      double f = A[(size_t)r * n + k] / A[(size_t)k * n + k];
+      if (f == 0.0) continue;
+      for (int j = k; j < n; ++j) A[(size_t)r * n + j] -= f * A[(size_t)k * n + j];
+      b[r] -= f * b[k];
+    }
+  }
+  std::vector<double> x(n);
+  for (int k = n - 1; k >= 0; --k) {
+    double v = b[k];
+    for (int j = k + 1; j < n; ++j) v -= A[(size_t)k * n + j] * x[j];
+    x[k] = v / A[(size_t)k * n + k];
+  }
+  coef.assign(m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    coef[i] = x[i] * s[i];
+    if (!std::isfinite(coef[i]) || std::fabs(coef[i]) > 1e6) return false;
+  }
+  return true;
+}
+
+// One amplitude update in the given precision.  On return c->amp holds the new
+// (possibly extrapolated) amplitudes; *dt2 = max |t2_new - t2_old|.
+int update_once(Ctx* c, int prec, bool allow_diis, bool* diis_fell_back, double* dt2) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  const double* t1 = c->amp;
+  const double* t2 = c->amp + d.ov;
+  TRY(compute_rhs(c, t1, t2, prec, true));
+  k_assemble_t2(c->amp_new + d.ov, c->err + d.ov, c->Rp, c->X, c->Y, c->Z, c->f, t2, 1, d, s);
+  k_update_t1(c->amp_new, c->err, c->R1, c->f, t1, d, s);
+  const long n2 = d.po * d.pv;
+  const int nb = reduce_blocks_for(n2);
+  k_maxabs_partials(c->partial, c->err + d.ov, n2, nb, s);
+  std::vector<double> hp(nb);
+  HIPTRY(hipMemcpyAsync(hp.data(), c->partial, nb * 8, hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  double mx = 0.0;
+  for (double v : hp) mx = std::max(mx, v);
+  *dt2 = mx;
+  *diis_fell_back = false;
+  if (c->diis_m >= 2 && allow_diis) {
+    const int m = c->diis_m;
+    const int slot = c->diis_next;
+    HIPTRY(hipMemcpyAsync(c->dvec + (long)slot * c->nvec, c->amp_new, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipMemcpyAsync(c->derr + (long)slot * c->nvec, c->err, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+    c->diis_count = std::min(c->diis_count + 1, m);
+    c->diis_next = (slot + 1) % m;
+    const int cnt = c->diis_count;
+    const int nbd = reduce_blocks_for(c->nvec);
+    k_multidot_partials(c->partial, c->derr, c->nvec, cnt, c->err, c->nvec, nbd, s);
+    k_sum_partials(c->scal, c->partial, nbd, cnt, s);
+    std::vector<double> row(cnt);
+    HIPTRY(hipMemcpyAsync(row.data(), c->scal, cnt * 8, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    for (int k = 0; k < cnt; ++k) {
+      c->B[(size_t)slot * m + k] = row[k];
+      c->B[(size_t)k * m + slot] = row[k];
+    }
+    if (cnt >= 2) {
+      std::vector<double> Bc((size_t)cnt * cnt), coef;
+      for (int i = 0; i < cnt; ++i)
+        for (int j = 0; j < cnt; ++j) Bc[(size_t)i * cnt + j] = c->B[(size_t)i * m + j];
+      if (diis_coefficients(Bc, cnt, coef)) {
+        HIPTRY(hipMemcpyAsync(c->dcoef, coef.data(), cnt * 8, hipMemcpyHostToDevice, s));
+        k_lincomb(c->amp, c->dvec, c->nvec, c->dcoef, cnt, c->nvec, s);
+        HIPTRY(hipGetLastError());
+        return CCSD_OK;
+      }
+      *diis_fell_back = true;
+    }
+  }
+  std::swap(c->amp, c->amp_new);
+  return CCSD_OK;
+}
+
+}  // namespace
+}  // namespace ccsd
+
+using namespace ccsd;
+
+extern "C" {
+
+int ccsd_default_options(ccsd_options* o) {
+  if (!o) return CCSD_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->precision = CCSD_PREC_FP64;
+  o->switch_threshold = 1e-6;
+  o->final_fp64_sweeps = 3;
+  o->diis_dim = 8;
+  o->diis_start = 1;
+  o->recompute_v4 = 1;
+  o->verbose = 0;
+  o->device = 0;
+  o->hbm_budget_bytes = 0;
+  return CCSD_OK;
+}
+
+size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision) {
+  if (n_occ < 2 || n_virt < 2) return 0;
+  return plan_bytes(n_occ, n_virt, precision, 8);
+}
+
+int ccsd_create(const ccsd_problem* p, const ccsd_options* o, ccsd_ctx** out) {
+  if (!p || !out || !p->fock || !p->oooo || !p->ooov || !p->oovv || !p->ovov || !p->ovvv || !p->vvvv)
+    return CCSD_ERR_ARG;
+  *out = nullptr;
+  ccsd_ctx* h = new (std::nothrow) ccsd_ctx();
+  if (!h) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  const int O = p->n_occ, V = p->n_virt;
+  if (O < 2 || V < 2) { delete h; return CCSD_ERR_ARG; }
+  const long N = O + V;
+  c->h_f.assign(p->fock, p->fock + N * N);
+  c->d = make_dims(O, V);
+  int rc = check_fock(c);
+  if (rc == CCSD_OK) {
+    const Dims& d = c->d;
+    auto finite = [](const double* x, long n) {
+      for (long i = 0; i < n; ++i)
+        if (!std::isfinite(x[i])) return false;
+      return true;
+    };
+    if (!finite(p->oooo, d.po * d.po) || !finite(p->ooov, d.po * d.ov) || !finite(p->oovv, d.po * d.pv) ||
+        !finite(p->ovov, d.ov * d.ov) || !finite(p->ovvv, d.ov * d.pv) || !finite(p->vvvv, d.pv * d.pv))
+      rc = CCSD_ERR_NONFINITE;
+  }
+  if (rc == CCSD_OK) rc = init_device(c, o, O, V);
+  if (rc == CCSD_OK) rc = setup_from_host(c, p);
+  if (rc == CCSD_OK) rc = mp2_start(c);
+  if (rc != CCSD_OK) {
+    if (rc == CCSD_ERR_HIP && !c->last_error.empty()) fprintf(stderr, "ccsd_create: %s\n", c->last_error.c_str());
+    release(c);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CCSD_OK;
+}
+
+int ccsd_create_synthetic(int n_occ, int n_virt, uint64_t seed, double scale, const ccsd_options* o,
+                          ccsd_ctx** out) {
+  if (!out || n_occ < 2 || n_virt < 2 || !(scale > 0.0) || !std::isfinite(scale)) return CCSD_ERR_ARG;
+  *out = nullptr;
+  ccsd_ctx* h = new (std::nothrow) ccsd_ctx();
+  if (!h) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  c->h_f = synthetic_fock(n_occ, n_virt, seed);
+  c->d = make_dims(n_occ, n_virt);
+  int rc = check_fock(c);
+  if (rc == CCSD_OK) rc = init_device(c, o, n_occ, n_virt);
+  if (rc == CCSD_OK) rc = setup_synthetic(c, seed, scale);
+  if (rc == CCSD_OK) rc = mp2_start(c);
+  if (rc != CCSD_OK) {
+    if (rc == CCSD_ERR_HIP && !c->last_error.empty())
+      fprintf(stderr, "ccsd_create_synthetic: %s\n", c->last_error.c_str());
+    release(c);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CCSD_OK;
+}
+
+void ccsd_destroy(ccsd_ctx* h) {
+  if (!h) return;
+  release(&h->c);
+  delete h;
+}
+
+int ccsd_reset(ccsd_ctx* h) {
+  if (!h) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  diis_reset(c);
+  c->solved = false;
+  return mp2_start(c);
+}
+
+int ccsd_solve(ccsd_ctx* h, double tol, int max_iter, ccsd_result* out) {
+  if (!h || !(tol >= 0.0) || max_iter < 1) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  ccsd_result r;
+  std::memset(&r, 0, sizeof(r));
+  double E;
+  TRY(device_energy(c, c->amp, c->amp + d.ov, &E));
+  if (!std::isfinite(E)) return CCSD_ERR_NONFINITE;
+  c->energies.assign(1, E);
+  r.e_mp2 = E;
+  c->e_mp2 = E;
+  diis_reset(c);
+  int prec = c->opt.precision;
+  int fp64_after_switch = (prec == CCSD_PREC_FP64) ? c->opt.final_fp64_sweeps : 0;
+  bool converged = false;
+  int rc = CCSD_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  int it = 0;
+  double dE = 0.0, dt2 = 0.0;
+  while (it < max_iter) {
+    const double E_old = E;
+    auto ti = std::chrono::steady_clock::now();
+    bool fell = false;
+    rc = update_once(c, prec, it + 1 >= c->opt.diis_start, &fell, &dt2);
+    if (rc != CCSD_OK) break;
+    if (fell) r.diis_fallbacks++;
+    if (prec == CCSD_PREC_FP64) { r.fp64_iterations++; fp64_after_switch++; }
+    else r.lowp_iterations++;
+    rc = device_energy(c, c->amp, c->amp + d.ov, &E);
+    if (rc != CCSD_OK) break;
+    ++it;
+    c->energies.push_back(E);
+    dE = std::fabs(E - E_old);
+    if (c->opt.verbose) {
+      double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ti).count();
+      printf("Energi: %.12f Steg: %d Time/iter: %.6f%s\n", E, it + 1, s, prec == CCSD_PREC_FP64 ? "" : " (mixed)");
+      fflush(stdout);
+    }
+    if (!std::isfinite(E)) { rc = CCSD_ERR_NONFINITE; break; }
+    if (prec != CCSD_PREC_FP64) {
+      if (dE < std::max(c->opt.switch_threshold, tol)) {
+        prec = CCSD_PREC_FP64;  // refine in FP64 from here on
+        fp64_after_switch = 0;
+        diis_reset(c);
+      }
+      continue;
+    }
+    if (dE < tol && fp64_after_switch >= std::min(c->opt.final_fp64_sweeps, r.fp64_iterations)) {
+      converged = true;
+      break;
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  r.e_corr = E;
+  r.iterations = it;
+  r.converged = converged ? 1 : 0;
+  r.last_delta_e = dE;
+  r.max_delta_t2 = dt2;
+  r.seconds_total = std::chrono::duration<double>(t1 - t0).count();
+  r.seconds_per_iter = it ? r.seconds_total / it : 0.0;
+  c->e_last = E;
+  c->solved = converged;
+  if (out) *out = r;
+  if (c->opt.verbose) {
+    printf("CCSD correction [au] = %.12f\n", E);
+    fflush(stdout);
+  }
+  if (rc != CCSD_OK) return rc;
+  return converged ? CCSD_OK : CCSD_ERR_NOT_CONVERGED;
+}
+
+int ccsd_iterate(ccsd_ctx* h, int n, int precision, double* energies) {
+  if (!h || n < 0 || precision < CCSD_PREC_FP64 || precision > CCSD_PREC_BF16X3) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  for (int k = 0; k < n; ++k) {
+    bool fell = false;
+    double dt2 = 0.0;
+    TRY(update_once(c, precision, true, &fell, &dt2));
+    double E;
+    TRY(device_energy(c, c->amp, c->amp + d.ov, &E));
+    c->energies.push_back(E);
+    if (energies) energies[k] = E;
+    if (!std::isfinite(E)) return CCSD_ERR_NONFINITE;
+  }
+  return CCSD_OK;
+}
+
+int ccsd_residual(ccsd_ctx* h, const double* t1, const double* t2, double* r1, double* r2) {
+  if (!h || !t1 || !t2 || !r1 || !r2) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  HIPTRY(hipMemcpyAsync(c->amp_new, t1, d.ov * 8, hipMemcpyHostToDevice, s));
+  HIPTRY(hipMemcpyAsync(c->amp_new + d.ov, t2, d.po * d.pv * 8, hipMemcpyHostToDevice, s));
+  TRY(compute_rhs(c, c->amp_new, c->amp_new + d.ov, CCSD_PREC_FP64, false));
+  k_assemble_t2(c->err + d.ov, nullptr, c->Rp, c->X, c->Y, c->Z, c->f, nullptr, 0, d, s);
+  HIPTRY(hipMemcpyAsync(r1, c->R1, d.ov * 8, hipMemcpyDeviceToHost, s));
+  HIPTRY(hipMemcpyAsync(r2, c->err + d.ov, d.po * d.pv * 8, hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  return CCSD_OK;
+}
+
+int ccsd_energy(ccsd_ctx* h, const double* t1, const double* t2, double* e) {
+  if (!h || !t1 || !t2 || !e) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  HIPTRY(hipMemcpyAsync(c->amp_new, t1, d.ov * 8, hipMemcpyHostToDevice, c->stream));
+  HIPTRY(hipMemcpyAsync(c->amp_new + d.ov, t2, d.po * d.pv * 8, hipMemcpyHostToDevice, c->stream));
+  return device_energy(c, c->amp_new, c->amp_new + d.ov, e);
+}
+
+int ccsd_get_amplitudes(ccsd_ctx* h, double* t1, double* t2) {
+  if (!h || !t1 || !t2) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  HIPTRY(hipMemcpyAsync(t1, c->amp, d.ov * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipMemcpyAsync(t2, c->amp + d.ov, d.po * d.pv * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(hipStreamSynchronize(c->stream));
+  return CCSD_OK;
+}
+
+int ccsd_set_amplitudes(ccsd_ctx* h, const double* t1, const double* t2) {
+  if (!h || !t1 || !t2) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  HIPTRY(hipSetDevice(c->device));
+  const Dims& d = c->d;
+  for (long i = 0; i < d.ov; ++i)
+    if (!std::isfinite(t1[i])) return CCSD_ERR_NONFINITE;
+  for (long i = 0; i < d.po * d.pv; ++i)
+    if (!std::isfinite(t2[i])) return CCSD_ERR_NONFINITE;
+  HIPTRY(hipMemcpyAsync(c->amp, t1, d.ov * 8, hipMemcpyHostToDevice, c->stream));
+  HIPTRY(hipMemcpyAsync(c->amp + d.ov, t2, d.po * d.pv * 8, hipMemcpyHostToDevice, c->stream));
+  HIPTRY(hipStreamSynchronize(c->stream));
+  diis_reset(c);
+  c->solved = false;
+  return CCSD_OK;
+}
+
+int ccsd_energy_trace(ccsd_ctx* h, double* out, int cap) {
+  if (!h) return CCSD_ERR_ARG;
+  const auto& e = h->c.energies;
+  if (out)
+    for (int i = 0; i < std::min<int>(cap, (int)e.size()); ++i) out[i] = e[i];
+  return (int)e.size();
+}
+
+int ccsd_last_timings(ccsd_ctx* h, double* ms, int cap) {
+  if (!h || !ms) return CCSD_ERR_ARG;
+  int n = std::min(cap, 5);
+  for (int i = 0; i < n; ++i) ms[i] = h->c.timings_ms[i];
+  return n;
+}
+
+long ccsd_debug_get(ccsd_ctx* h, const char* name, double* out, long cap) {
+  if (!h || !name) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  const Dims& d = c->d;
+  const long O = d.O, V = d.V;
+  struct Item { const char* n; const double* p; long cnt; };
+  const Item items[] = {
+      {"Fme", c->Fme, d.ov},           {"Fmi", c->Fmi, O * O},          {"Fae", c->Fae, V * V},
+      {"W1", c->W1, d.po * d.po},      {"W2", c->W2, d.po * d.ov},      {"W3", c->W3, O * d.po * V},
+      {"WT", c->WT, d.ov * d.ov},      {"X", c->X, d.ov * d.ov},        {"Y", c->Y, d.po * V * V},
+      {"Z", c->Z, O * O * d.pv},       {"Rp", c->Rp, d.po * d.pv},      {"R1", c->R1, d.ov},
+      {"tau", c->tau, d.po * d.pv},    {"Tring", c->Tring, d.ov * d.ov}, {"Tv", c->Tv, d.po * V * V},
+      {"T2u", c->T2u, O * O * d.pv},
+  };
+  for (const Item& it : items) {
+    if (std::strcmp(it.n, name) != 0) continue;
+    if (out) {
+      if (cap < it.cnt) return CCSD_ERR_ARG;
+      HIPTRY(hipSetDevice(c->device));
+      HIPTRY(hipMemcpyAsync(out, it.p, it.cnt * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPTRY(hipStreamSynchronize(c->stream));
+    }
+    return it.cnt;
+  }
+  return CCSD_ERR_ARG;
+}
+
+int ccsd_backward(ccsd_ctx* h, double* dE_dfock, double* dE_doooo, double* dE_dooov, double* dE_doovv,
+                  double* dE_dovov, double* dE_dovvv, double* dE_dvvvv, ccsd_grad_info* info) {
+  if (!h || !dE_dfock || !dE_doooo || !dE_dooov || !dE_doovv || !dE_dovov || !dE_dovvv || !dE_dvvvv)
+    return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->solved) return CCSD_ERR_STATE;
+  HIPTRY(hipSetDevice(c->device));
+  GradOut g{dE_dfock, dE_doooo, dE_dooov, dE_doovv, dE_dovov, dE_dovvv, dE_dvvvv};
+  return adjoint_run(c, g, info);
+}
+
+const char* ccsd_strerror(int code) {
+  switch (code) {
+    case CCSD_OK: return "success";
+    case CCSD_ERR_ARG: return "invalid argument";
+    case CCSD_ERR_HIP: return "HIP runtime error";
+    case CCSD_ERR_BUDGET: return "plan exceeds the HBM budget";
+    case CCSD_ERR_NONFINITE: return "non-finite value in inputs or iteration";
+    case CCSD_ERR_NOT_CONVERGED: return "not converged within max_iter";
+    case CCSD_ERR_DENOMINATOR: return "vanishing orbital-energy denominator";
+    case CCSD_ERR_NO_DEVICE: return "no HIP device";
+    case CCSD_ERR_STATE: return "invalid call order";
+    default: return "unknown error";
+  }
+}
+
+int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const double* A, const double* B,
+                       double beta, double* C, int precision) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return CCSD_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCSD_ERR_NO_DEVICE;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CCSD_ERR_HIP;
+  double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  void* ws = nullptr;
+  int rc = CCSD_OK;
+  const size_t wsb = 64u << 20;
+  if (hipMalloc(&dA, (size_t)M * K * 8) != hipSuccess || hipMalloc(&dB, (size_t)K * N * 8) != hipSuccess ||
+      hipMalloc(&dC, (size_t)M * N * 8) != hipSuccess || hipMalloc(&ws, wsb) != hipSuccess) {
+    rc = CCSD_ERR_HIP;
+  }
+  if (rc == CCSD_OK) {
+    if (precision == CCSD_PREC_FP64) {
+      (void)hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice);
+      (void)hipMemcpy(dB, B, (size_t)K * N * 8, hipMemcpyHostToDevice);
+      (void)hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice);
+      GemmDesc<double> g;
+      g.A = dA; g.B = dB; g.C = dC; g.M = M; g.N = N; g.K = K;
+      g.sAm = ta ? 1 : K; g.sAk = ta ? M : 1;
+      g.sBk = tb ? 1 : N; g.sBn = tb ? K : 1;
+      g.sCm = N; g.sCn = 1;
+      g.alpha = alpha; g.beta = beta;
+      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s) != hipSuccess) rc = CCSD_ERR_HIP;
+    } else {
+      // low-precision path wants A as [M][K] and B as [N][K]
+      std::vector<double> Ak((size_t)M * K), Bk((size_t)N * K);
+      for (long m = 0; m < M; ++m)
+        for (long k = 0; k < K; ++k) Ak[m * K + k] = ta ? A[k * M + m] : A[m * K + k];
+      for (long n = 0; n < N; ++n)
+        for (long k = 0; k < K; ++k) Bk[n * K + k] = tb ? B[n * K + k] : B[k * N + n];
+      (void)hipMemcpy(dA, Ak.data(), (size_t)M * K * 8, hipMemcpyHostToDevice);
+      (void)hipMemcpy(dB, Bk.data(), (size_t)K * N * 8, hipMemcpyHostToDevice);
+      (void)hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice);
+      rc = lowprec_gemm_nt(dA, dB, dC, M, N, K, alpha, beta, precision, s);
+    }
+  }
+  if (rc == CCSD_OK) {
+    if (hipStreamSynchronize(s) != hipSuccess || hipMemcpy(C, dC, (size_t)M * N * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = CCSD_ERR_HIP;
+  }
+  (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(ws);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+}  // extern "C"

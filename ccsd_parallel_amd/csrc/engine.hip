@@ -1,0 +1,389 @@
+// CCSD right-hand side on the GPU: one amplitude update expressed as MFMA GEMMs
+// over packed antisymmetric layouts plus bandwidth kernels.
+//
+// Factorisation follows CCSD_Memory_optimized (ccsd_memory_optimized.cpp):
+//   F_me, F_mi (+F_me t_i^e, Fill_F2 :1591-1629), F_ae (-F_me t_m^a, Fill_F3 :1631-1667),
+//   W_mnij with 1/2 tau (W_1, :659-1029), W_mnie (W_3), W_mbij with 1/2 <mb||ef> tau
+//   (W_2, :1047-1295), W_mbej (W_4, Fill_W4_MPI :4781-5089), ring (Fill_W5 :6201-6882),
+//   bare pp ladder 1/2 <ab||cd> tau and the rest of Fill_t2_new (:2001-2458).
+// The reference evaluates each term as Armadillo dot products over compact RHF
+// slices; here every O(N^5-6) term is one strided/batched GEMM call.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+#include "lowprec.hpp"
+
+namespace ccsd {
+
+namespace {
+
+GemmDesc<double> G64(const double* A, long sAm, long sAk, const double* B, long sBk, long sBn, double* C, long sCm,
+                     long sCn, long M, long N, long K, double alpha, double beta) {
+  GemmDesc<double> g;
+  g.A = A; g.B = B; g.C = C;
+  g.M = (int)M; g.N = (int)N; g.K = (int)K;
+  g.sAm = sAm; g.sAk = sAk; g.sBk = sBk; g.sBn = sBn; g.sCm = sCm; g.sCn = sCn;
+  g.alpha = alpha; g.beta = beta;
+  return g;
+}
+
+#define TRY(x)                         \
+  do {                                 \
+    int _rc = (x);                     \
+    if (_rc != CCSD_OK) return _rc;    \
+  } while (0)
+
+#define HIPTRY(x)                                      \
+  do {                                                 \
+    hipError_t _e = (x);                               \
+    if (_e != hipSuccess) {                            \
+      c->last_error = hipGetErrorString(_e);           \
+      return CCSD_ERR_HIP;                             \
+    }                                                  \
+  } while (0)
+
+}  // namespace
+
+size_t plan_bytes(int O, int V, int precision, int diis_m) {
+  Dims d = make_dims(O, V);
+  const long N = O + V;
+  const long VV = (long)V * V;
+  long n = 0;
+  // static
+  n += N * N + d.ov;
+  n += d.po * d.po + d.po * d.ov + d.po * d.pv + d.ov * d.pv + d.pv * d.pv;
+  n += 2 * d.ov * d.ov + d.ov * VV + (long)O * O * d.pv + d.po * VV;
+  // amplitudes
+  long nvec = d.ov + d.po * d.pv;
+  n += 3 * nvec + d.po * d.pv + 2L * O * O * d.pv + d.po * VV + d.ov * d.ov;
+  // intermediates
+  n += d.ov + (long)O * O + 2 * VV + d.po * d.po + d.po * O * O + (long)O * d.po * V + (long)O * O * O * V;
+  n += d.po * d.ov + (long)O * O * d.ov + 2 * d.ov * d.ov + d.po * VV + (long)O * O * d.pv + d.po * d.pv + d.ov;
+  // DIIS
+  if (diis_m >= 2) n += 2L * diis_m * nvec + diis_m;
+  // temporary ovov at setup shares nothing: count it
+  n += d.ov * d.ov;
+  size_t bytes = (size_t)n * sizeof(double);
+  bytes += 64ull << 20;  // GEMM workspace
+  bytes += 1ull << 20;   // reductions
+  bytes += lowprec_plan_bytes(d, precision);
+  return bytes;
+}
+
+static int dalloc(Ctx* c, double** p, long n) {
+  if (n <= 0) n = 1;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, (size_t)n * sizeof(double));
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  c->allocs.push_back(q);
+  c->bytes += (size_t)n * sizeof(double);
+  *p = (double*)q;
+  return CCSD_OK;
+}
+
+int plan_and_allocate(Ctx* c, int precision) {
+  const Dims& d = c->d;
+  const long O = d.O, V = d.V, N = O + V, VV = V * V;
+  size_t need = plan_bytes(d.O, d.V, precision, c->opt.diis_dim);
+  size_t budget = c->opt.hbm_budget_bytes;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    HIPTRY(hipMemGetInfo(&fr, &tot));
+    budget = (size_t)(0.9 * (double)fr);
+  }
+  if (need > budget) {
+    c->last_error = "plan needs " + std::to_string(need) + " bytes, budget " + std::to_string(budget);
+    return CCSD_ERR_BUDGET;
+  }
+  TRY(dalloc(c, &c->f, N * N));
+  TRY(dalloc(c, &c->fov, d.ov));
+  TRY(dalloc(c, &c->oooo, d.po * d.po));
+  TRY(dalloc(c, &c->ooov, d.po * d.ov));
+  TRY(dalloc(c, &c->oovv, d.po * d.pv));
+  TRY(dalloc(c, &c->ovvv, d.ov * d.pv));
+  TRY(dalloc(c, &c->vvvv, d.pv * d.pv));
+  TRY(dalloc(c, &c->G, d.ov * d.ov));
+  TRY(dalloc(c, &c->Doovv, d.ov * d.ov));
+  TRY(dalloc(c, &c->ovvv_u, d.ov * VV));
+  TRY(dalloc(c, &c->oovv_ou, O * O * d.pv));
+  TRY(dalloc(c, &c->oovv_vu, d.po * VV));
+  c->nvec = d.ov + d.po * d.pv;
+  TRY(dalloc(c, &c->amp, c->nvec));
+  TRY(dalloc(c, &c->amp_new, c->nvec));
+  TRY(dalloc(c, &c->err, c->nvec));
+  TRY(dalloc(c, &c->tau, d.po * d.pv));
+  TRY(dalloc(c, &c->T2u, O * O * d.pv));
+  TRY(dalloc(c, &c->tauu, O * O * d.pv));
+  TRY(dalloc(c, &c->Tv, d.po * VV));
+  TRY(dalloc(c, &c->Tring, d.ov * d.ov));
+  TRY(dalloc(c, &c->Fme, d.ov));
+  TRY(dalloc(c, &c->Fmi, O * O));
+  TRY(dalloc(c, &c->Fae, VV));
+  TRY(dalloc(c, &c->Fae_t1, VV));
+  TRY(dalloc(c, &c->W1, d.po * d.po));
+  TRY(dalloc(c, &c->Q, d.po * O * O));
+  TRY(dalloc(c, &c->W3, O * d.po * V));
+  TRY(dalloc(c, &c->W3u, O * O * O * V));
+  TRY(dalloc(c, &c->W2, d.po * d.ov));
+  TRY(dalloc(c, &c->U, O * O * d.ov));
+  TRY(dalloc(c, &c->WT, d.ov * d.ov));
+  TRY(dalloc(c, &c->X, d.ov * d.ov));
+  TRY(dalloc(c, &c->Y, d.po * VV));
+  TRY(dalloc(c, &c->Z, O * O * d.pv));
+  TRY(dalloc(c, &c->Rp, d.po * d.pv));
+  TRY(dalloc(c, &c->R1, d.ov));
+  c->diis_m = c->opt.diis_dim >= 2 ? c->opt.diis_dim : 0;
+  if (c->diis_m) {
+    TRY(dalloc(c, &c->dvec, (long)c->diis_m * c->nvec));
+    TRY(dalloc(c, &c->derr, (long)c->diis_m * c->nvec));
+    TRY(dalloc(c, &c->dcoef, c->diis_m));
+    c->B.assign((size_t)c->diis_m * c->diis_m, 0.0);
+  }
+  TRY(dalloc(c, &c->partial, 1 << 16));
+  TRY(dalloc(c, &c->scal, 64));
+  double* ws = nullptr;
+  TRY(dalloc(c, &ws, (64l << 20) / 8));
+  c->ws.ptr = ws;
+  c->ws.bytes = 64ull << 20;
+  TRY(lowprec_allocate(c, precision));
+  for (auto& e : c->ev) HIPTRY(hipEventCreate(&e));
+  (void)N;
+  return CCSD_OK;
+}
+
+// Derived static layouts from the packed blocks (ovov consumed into G).
+static int finish_static(Ctx* c, const double* ovov_dev) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  k_build_G(c->G, ovov_dev, d, s);
+  k_build_Doovv(c->Doovv, c->oovv, d, s);
+  k_build_ovvv_u(c->ovvv_u, c->ovvv, d, s);
+  k_build_oovv_ou(c->oovv_ou, c->oovv, d, s);
+  k_build_oovv_vu(c->oovv_vu, c->oovv, d, s);
+  std::vector<double> fov((size_t)d.ov);
+  const long N = d.O + d.V;
+  for (long i = 0; i < d.O; ++i)
+    for (long a = 0; a < d.V; ++a) fov[i * d.V + a] = c->h_f[i * N + d.O + a];
+  HIPTRY(hipMemcpyAsync(c->fov, fov.data(), fov.size() * 8, hipMemcpyHostToDevice, s));
+  HIPTRY(hipMemcpyAsync(c->f, c->h_f.data(), c->h_f.size() * 8, hipMemcpyHostToDevice, s));
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipStreamSynchronize(s));
+  return CCSD_OK;
+}
+
+int setup_from_host(Ctx* c, const ccsd_problem* p) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  auto up = [&](double* dst, const double* src, long n) -> int {
+    HIPTRY(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    return CCSD_OK;
+  };
+  TRY(up(c->oooo, p->oooo, d.po * d.po));
+  TRY(up(c->ooov, p->ooov, d.po * d.ov));
+  TRY(up(c->oovv, p->oovv, d.po * d.pv));
+  TRY(up(c->ovvv, p->ovvv, d.ov * d.pv));
+  TRY(up(c->vvvv, p->vvvv, d.pv * d.pv));
+  // ovov is staged in X (same ov x ov shape) and consumed into G.
+  TRY(up(c->X, p->ovov, d.ov * d.ov));
+  return finish_static(c, c->X);
+}
+
+int setup_synthetic(Ctx* c, uint64_t seed, double scale) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  k_synth_block(c->oooo, 0, 0, d.po, d, seed, scale, s);
+  k_synth_block(c->ooov, 1, 0, d.po, d, seed, scale, s);
+  k_synth_block(c->oovv, 2, 0, d.po, d, seed, scale, s);
+  k_synth_block(c->X, 3, 0, d.ov, d, seed, scale, s);
+  k_synth_block(c->ovvv, 4, 0, d.ov, d, seed, scale, s);
+  k_synth_block(c->vvvv, 5, 0, d.pv, d, seed, scale, s);
+  HIPTRY(hipGetLastError());
+  return finish_static(c, c->X);
+}
+
+// All right-hand-side pieces at amplitudes (t1, t2) (device pointers):
+//   Rp  = <ab||ij> + pp ladder + hh ladder           (packed)
+//   X   = ring GEMM (P(ij)P(ab) applied at assembly)
+//   Y   = P(ab) pieces, Z = P(ij) pieces, R1 = T1 right-hand side.
+int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_it) {
+  const Dims& d = c->d;
+  const long O = d.O, V = d.V, VV = V * V, po = d.po, pv = d.pv, ov = d.ov;
+  hipStream_t s = c->stream;
+  const bool lowp = prec != CCSD_PREC_FP64;
+  float ms_pp = 0, ms_ring = 0, ms_wq = 0, ms_w2 = 0, ms_all = 0;
+  if (time_it) HIPTRY(hipEventRecord(c->ev[0], s));
+
+  // -- amplitude layouts
+  k_build_tau(c->tau, t2, t1, d, s);
+  k_unpack_occ(c->T2u, c->tauu, t2, c->tau, d, s);
+  k_unpack_vir(c->Tv, t2, d, s);
+  k_unpack_ring(c->Tring, t2, d, s);
+  if (lowp) TRY(lowprec_amplitudes(c, prec));
+
+  // -- F_me = f_me + <mn||ef> t_n^f
+  k_fme_init(c->Fme, c->f, d, s);
+  TRY(run_gemm(c, G64(c->Doovv, ov, 1, t1, 1, 0, c->Fme, 1, 0, ov, 1, ov, 1.0, 1.0)));
+
+  // -- W_mnij = <mn||ij> + 1/2 tau <mn||ef> + P(ij) t_j^e <mn||ie>
+  k_copy(c->W1, c->oooo, po * po, s);
+  TRY(run_gemm(c, G64(c->oovv, pv, 1, c->tau, 1, pv, c->W1, po, 1, po, po, pv, 1.0, 1.0)));
+  TRY(run_gemm(c, G64(c->ooov, V, 1, t1, 1, V, c->Q, O, 1, po * O, O, V, 1.0, 0.0)));
+  k_w1_finish(c->W1, c->Q, d, s);
+
+  // -- W_mnie (layout [i][mn][e]) = <mn||ie> + t_i^f <mn||fe>
+  k_w3_init(c->W3, c->ooov, d, s);
+  {
+    GemmDesc<double> g = G64(t1, V, 1, c->oovv_vu, V, 1, c->W3, po * V, 1, O, V, V, 1.0, 1.0);
+    g.nb1 = (int)po; g.sB1 = VV; g.sC1 = V;
+    TRY(run_gemm(c, g));
+  }
+  k_w3_unpack(c->W3u, c->W3, d, s);
+
+  // -- W_mbij (layout [ij][(m,b)]) = <mb||ij> + 1/2 tau <mb||ef> + P(ij) t_i^e <mb||ej>
+  k_copy(c->W2, c->ooov, po * ov, s);
+  if (time_it) HIPTRY(hipEventRecord(c->ev[1], s));
+  if (lowp) {
+    TRY(lowprec_w2(c, prec));
+  } else {
+    TRY(run_gemm(c, G64(c->tau, pv, 1, c->ovvv, 1, pv, c->W2, ov, 1, po, ov, pv, 1.0, 1.0)));
+  }
+  if (time_it) HIPTRY(hipEventRecord(c->ev[2], s));
+  {
+    GemmDesc<double> g = G64(t1, V, 1, c->G, ov, 1, c->U, O * ov, 1, O, V, V, 1.0, 0.0);
+    g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = m, b2 = j
+    g.sB1 = V * ov; g.sB2 = V;
+    g.sC1 = V; g.sC2 = ov;
+    TRY(run_gemm(c, g));
+  }
+  k_w2_finish(c->W2, c->U, d, s);
+
+  // -- W_mbej stored transposed: WT[(j,b),(m,e)]
+  {
+    // t_j^f <mb||ef>, batched over (m,b)
+    GemmDesc<double> g = G64(t1, V, 1, c->ovvv_u, 1, V, c->WT, V * ov, 1, O, V, V, 1.0, 0.0);
+    g.nb1 = (int)O; g.nb2 = (int)V;  // b1 = m, b2 = b
+    g.sB1 = V * VV; g.sB2 = VV;
+    g.sC1 = V; g.sC2 = ov;
+    TRY(run_gemm(c, g));
+  }
+  k_fae_t1_extract(c->Fae_t1, c->WT, d, s);  // F_ae t1 part = -sum_m (t <mb||ef>)_{j=m, b=a}
+  k_axpy(c->WT, c->G, 1.0, ov * ov, s);      // + <mb||ej>
+  {
+    // + t_n^b W_mnje, batched over (j, m)
+    GemmDesc<double> g = G64(t1, 1, V, c->W3u, V, 1, c->WT, ov, 1, V, V, O, 1.0, 1.0);
+    g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = j, b2 = m
+    g.sB1 = O * O * V; g.sB2 = O * V;
+    g.sC1 = V * ov; g.sC2 = V;
+    TRY(run_gemm(c, g));
+  }
+  if (time_it) HIPTRY(hipEventRecord(c->ev[3], s));
+  if (lowp) {
+    TRY(lowprec_wmbej(c, prec));
+  } else {
+    // + 1/2 t_jn^bf <mn||ef>
+    TRY(run_gemm(c, G64(c->Tring, ov, 1, c->Doovv, ov, 1, c->WT, ov, 1, ov, ov, ov, 0.5, 1.0)));
+  }
+  if (time_it) HIPTRY(hipEventRecord(c->ev[4], s));
+
+  // -- F_mi
+  k_fmi_init(c->Fmi, c->f, c->ooov, t1, d, s);
+  TRY(run_gemm(c, G64(c->oovv_ou, O * pv, 1, c->T2u, 1, O * pv, c->Fmi, O, 1, O, O, O * pv, 1.0, 1.0)));
+  k_fmi_finish(c->Fmi, c->Fme, t1, d, s);
+  // -- F_ae
+  k_fae_init(c->Fae, c->f, c->Fae_t1, c->Fme, t1, d, s);
+  TRY(run_gemm(c, G64(c->Tv, 1, V, c->oovv_vu, V, 1, c->Fae, V, 1, V, V, po * V, -1.0, 1.0)));
+
+  // -- packed residual: <ab||ij> + pp ladder + hh ladder
+  k_copy(c->Rp, c->oovv, po * pv, s);
+  if (time_it) HIPTRY(hipEventRecord(c->ev[5], s));
+  if (lowp) {
+    TRY(lowprec_ladder(c, prec));
+  } else {
+    TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, pv, 1, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0)));
+  }
+  if (time_it) HIPTRY(hipEventRecord(c->ev[6], s));
+  TRY(run_gemm(c, G64(c->W1, 1, po, c->tau, pv, 1, c->Rp, pv, 1, po, pv, po, 1.0, 1.0)));
+
+  // -- ring X = Tring * W_mbej
+  if (time_it) HIPTRY(hipEventRecord(c->ev[7], s));
+  if (lowp) {
+    TRY(lowprec_ring(c, prec));
+  } else {
+    TRY(run_gemm(c, G64(c->Tring, ov, 1, c->WT, 1, ov, c->X, ov, 1, ov, ov, ov, 1.0, 0.0)));
+  }
+  if (time_it) HIPTRY(hipEventRecord(c->ev[8], s));
+
+  // -- Y[ij][a][b] = t_ij^ae F_be - t_m^a W_mbij
+  TRY(run_gemm(c, G64(c->Tv, V, 1, c->Fae, 1, V, c->Y, V, 1, po * V, V, V, 1.0, 0.0)));
+  {
+    GemmDesc<double> g = G64(t1, 1, V, c->W2, V, 1, c->Y, V, 1, V, V, O, -1.0, 1.0);
+    g.nb1 = (int)po; g.sB1 = ov; g.sC1 = VV;
+    TRY(run_gemm(c, g));
+  }
+  // -- Z[i][j][ab] = -t_im^ab F_mj - t_i^e <je||ab>
+  {
+    GemmDesc<double> g = G64(c->Fmi, 1, O, c->T2u, pv, 1, c->Z, pv, 1, O, pv, O, -1.0, 0.0);
+    g.nb1 = (int)O; g.sB1 = O * pv; g.sC1 = O * pv;
+    TRY(run_gemm(c, g));
+  }
+  {
+    GemmDesc<double> g = G64(t1, V, 1, c->ovvv, pv, 1, c->Z, O * pv, 1, O, pv, V, -1.0, 1.0);
+    g.nb1 = (int)O; g.sB1 = V * pv; g.sC1 = pv;
+    TRY(run_gemm(c, g));
+  }
+
+  // -- T1 right-hand side
+  k_r1_init(c->R1, c->f, t1, d, s);
+  {
+    // -1/2 tau_im^ef <ma||ef>, summed over m
+    GemmDesc<double> g = G64(c->tauu, O * pv, 1, c->ovvv, 1, pv, c->R1, V, 1, O, V, pv, -1.0, 1.0);
+    g.nb1 = (int)O; g.sA1 = pv; g.sB1 = V * pv; g.reduce_batch = true;
+    TRY(run_gemm(c, g));
+  }
+  TRY(run_gemm(c, G64(c->W3, po * V, 1, c->Tv, V, 1, c->R1, V, 1, O, V, po * V, 1.0, 1.0)));  // -1/2 t_mn^ae W_mnie
+  TRY(run_gemm(c, G64(c->Tring, ov, 1, c->Fme, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0)));       // t_im^ae F_me
+  TRY(run_gemm(c, G64(c->G, ov, 1, t1, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0)));               // -t_n^f <na||if>
+  TRY(run_gemm(c, G64(c->Fmi, 1, O, t1, V, 1, c->R1, V, 1, O, V, O, -1.0, 1.0)));               // -t_m^a F_mi
+
+  if (time_it) {
+    HIPTRY(hipEventRecord(c->ev[9], s));
+    HIPTRY(hipEventSynchronize(c->ev[9]));
+    HIPTRY(hipEventElapsedTime(&ms_w2, c->ev[1], c->ev[2]));
+    HIPTRY(hipEventElapsedTime(&ms_wq, c->ev[3], c->ev[4]));
+    HIPTRY(hipEventElapsedTime(&ms_pp, c->ev[5], c->ev[6]));
+    HIPTRY(hipEventElapsedTime(&ms_ring, c->ev[7], c->ev[8]));
+    HIPTRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[9]));
+    c->timings_ms[0] = ms_pp;
+    c->timings_ms[1] = ms_ring;
+    c->timings_ms[2] = ms_wq;
+    c->timings_ms[3] = ms_w2;
+    c->timings_ms[4] = ms_all - ms_pp - ms_ring - ms_wq - ms_w2;
+  }
+  HIPTRY(hipGetLastError());
+  return CCSD_OK;
+}
+
+// E = f_ia t_i^a + sum_{i<j,a<b} <ij||ab> tau_ij^ab  (Calc_Energy, :592-648)
+int device_energy(Ctx* c, const double* t1, const double* t2, double* e) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  k_build_tau(c->tau, t2, t1, d, s);
+  const long n2 = d.po * d.pv;
+  const int nb2 = reduce_blocks_for(n2), nb1 = reduce_blocks_for(d.ov);
+  k_dot_partials(c->partial, c->oovv, c->tau, n2, nb2, s);
+  k_dot_partials(c->partial + nb2, c->fov, t1, d.ov, nb1, s);
+  k_sum_partials(c->scal, c->partial, nb2, 1, s);
+  k_sum_partials(c->scal + 1, c->partial + nb2, nb1, 1, s);
+  double h[2];
+  HIPTRY(hipMemcpyAsync(h, c->scal, 16, hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  *e = h[0] + h[1];
+  return CCSD_OK;
+}
+
+}  // namespace ccsd

@@ -1,0 +1,78 @@
+// Engine context shared by the forward driver, the adjoint and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ccsd_amd.h"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+namespace ccsd {
+
+struct LowPrec;  // FP32 / BF16 operand copies for the mixed-precision GEMMs
+
+struct Ctx {
+  Dims d{};
+  ccsd_options opt{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t bytes = 0;
+  std::vector<void*> allocs;
+
+  // static data (FP64)
+  double *f = nullptr, *fov = nullptr;
+  double *oooo = nullptr, *ooov = nullptr, *oovv = nullptr, *ovvv = nullptr, *vvvv = nullptr;
+  double *G = nullptr, *Doovv = nullptr, *ovvv_u = nullptr, *oovv_ou = nullptr, *oovv_vu = nullptr;
+  std::vector<double> h_f;  // host copy of the Fock matrix
+
+  // amplitudes: [t1 (ov) | t2 packed (po*pv)]
+  long nvec = 0;
+  double *amp = nullptr, *amp_new = nullptr, *err = nullptr;
+  double *tau = nullptr, *T2u = nullptr, *tauu = nullptr, *Tv = nullptr, *Tring = nullptr;
+
+  // intermediates
+  double *Fme = nullptr, *Fmi = nullptr, *Fae = nullptr, *Fae_t1 = nullptr;
+  double *W1 = nullptr, *Q = nullptr, *W3 = nullptr, *W3u = nullptr, *W2 = nullptr, *U = nullptr;
+  double *WT = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr, *Rp = nullptr, *R1 = nullptr;
+
+  // DIIS
+  int diis_m = 0, diis_count = 0, diis_next = 0;
+  double *dvec = nullptr, *derr = nullptr, *dcoef = nullptr;
+  std::vector<double> B;  // diis_m x diis_m Gram matrix (host, FP64)
+
+  // reductions / scratch
+  double* partial = nullptr;
+  double* scal = nullptr;  // small device scalars
+  GemmWorkspace ws;
+
+  LowPrec* lp = nullptr;
+
+  // state
+  bool solved = false;
+  double e_mp2 = 0.0, e_last = 0.0;
+  std::vector<double> energies;
+  double timings_ms[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[12] = {};
+  std::string last_error;
+};
+
+int setup_from_host(Ctx* c, const ccsd_problem* p);
+int setup_synthetic(Ctx* c, uint64_t seed, double scale);
+int plan_and_allocate(Ctx* c, int precision);
+int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool time_it);
+int device_energy(Ctx* c, const double* t1, const double* t2, double* e);
+size_t plan_bytes(int O, int V, int precision, int diis_m);
+
+// GEMM helper with error propagation
+template <typename T>
+inline int run_gemm(Ctx* c, const GemmDesc<T>& g) {
+  hipError_t e = gemm<T>(g, c->ws, c->stream);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  return CCSD_OK;
+}
+
+}  // namespace ccsd

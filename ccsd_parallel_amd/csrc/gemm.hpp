@@ -1,0 +1,49 @@
+// Strided, batched MFMA GEMM engine for CDNA4 (gfx950).
+//
+//   C[b](m,n) = alpha * sum_k A[b](m,k) * B[b](k,n) + beta * C[b](m,n)
+//
+// Every CCSD contraction on the hot path (pp/hh ladders, ring, W/F builds) is
+// expressed as one call of this engine over the packed antisymmetric layouts;
+// the element addresses are
+//   A: A + b1*sA1 + b2*sA2 + m*sAm + k*sAk   (and likewise for B, C)
+// with the batch index z = b1*nb2 + b2.  When ``reduce_batch`` is set, all
+// batches sum into one C (the batch strides of C are ignored).  Split-K and
+// batch reductions go through a workspace and a second deterministic pass, so
+// results are bit-reproducible run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ccsd {
+
+template <typename T>
+struct GemmDesc {
+  const T* A = nullptr;
+  const T* B = nullptr;
+  T* C = nullptr;
+  int M = 0, N = 0, K = 0;
+  long sAm = 0, sAk = 0, sBk = 0, sBn = 0, sCm = 0, sCn = 0;
+  int nb1 = 1, nb2 = 1;
+  long sA1 = 0, sA2 = 0, sB1 = 0, sB2 = 0, sC1 = 0, sC2 = 0;
+  T alpha = 1, beta = 0;
+  bool reduce_batch = false;
+};
+
+// Scratch the engine may use for split-K / batch-reduce partials.
+struct GemmWorkspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Launch statistics of the last call (for tests and the profiler notes).
+struct GemmLaunchInfo {
+  int tiles_m = 0, tiles_n = 0, batches = 0, splits = 0, config = 0;
+};
+
+template <typename T>
+hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t stream,
+                GemmLaunchInfo* info = nullptr, int force_config = -1, int force_split = 0);
+
+// Plain FP64 GEMV-like helper is just gemm with N=1.
+
+}  // namespace ccsd

@@ -1,0 +1,583 @@
+// Bandwidth-bound CCSD kernels (FP64).  All are grid-stride loops over the
+// destination so every write is coalesced; gathers from the packed
+// antisymmetric storage carry the permutation sign.
+#include "kernels.hpp"
+#include <algorithm>
+
+namespace ccsd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline int grid_for(long n) {
+  long b = (n + kThreads - 1) / kThreads;
+  return (int)std::max<long>(1, std::min<long>(b, 1L << 20));
+}
+
+__device__ __forceinline__ long pidx(long p, long q) { return q * (q - 1) / 2 + p; }
+
+// Pair index -> (p, q), p < q.
+__device__ __forceinline__ void pair_decode(long k, int& p, int& q) {
+  long qq = (long)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+  while (qq * (qq - 1) / 2 > k) --qq;
+  while ((qq + 1) * qq / 2 <= k) ++qq;
+  q = (int)qq;
+  p = (int)(k - qq * (qq - 1) / 2);
+}
+
+// Signed gather from a packed antisymmetric pair axis: x(p,q) with x(q,p) = -x(p,q).
+__device__ __forceinline__ double sgn_pair(int p, int q, long& k) {
+  if (p == q) { k = 0; return 0.0; }
+  if (p < q) { k = pidx(p, q); return 1.0; }
+  k = pidx(q, p);
+  return -1.0;
+}
+
+#define GRID_LOOP(idx, n) for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (n); idx += (long)gridDim.x * blockDim.x)
+
+__global__ void copy_kernel(double* dst, const double* src, long n) { GRID_LOOP(i, n) dst[i] = src[i]; }
+__global__ void axpy_kernel(double* y, const double* x, double a, long n) { GRID_LOOP(i, n) y[i] += a * x[i]; }
+__global__ void fill_kernel(double* y, double v, long n) { GRID_LOOP(i, n) y[i] = v; }
+__global__ void to_f32_kernel(float* y, const double* x, long n) { GRID_LOOP(i, n) y[i] = (float)x[i]; }
+
+// Round-to-nearest-even FP32 -> BF16 bits.
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan passthrough
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__global__ void split_bf16_kernel(uint16_t* hi, uint16_t* lo, const double* x, long n) {
+  GRID_LOOP(i, n) {
+    float v = (float)x[i];
+    uint16_t h = f32_to_bf16_bits(v);
+    float r = (float)(x[i] - (double)bf16_bits_to_f32(h));
+    hi[i] = h;
+    lo[i] = f32_to_bf16_bits(r);
+  }
+}
+
+// ---------------------------------------------------------------- static layouts
+__global__ void build_G_kernel(double* G, const double* ovov, Dims d) {
+  const long n = d.ov * d.ov;
+  GRID_LOOP(idx, n) {
+    long r = idx / d.ov, c = idx % d.ov;  // r = (m,e), c = (j,b)
+    int m = (int)(r / d.V), e = (int)(r % d.V), j = (int)(c / d.V), b = (int)(c % d.V);
+    // <mb||ej> = -<mb||je>
+    G[idx] = -ovov[((long)m * d.V + b) * d.ov + (long)j * d.V + e];
+  }
+}
+
+__global__ void build_Doovv_kernel(double* D, const double* oovv, Dims d) {
+  const long n = d.ov * d.ov;
+  GRID_LOOP(idx, n) {
+    long r = idx / d.ov, c = idx % d.ov;  // r = (m,e), c = (n,f)
+    int m = (int)(r / d.V), e = (int)(r % d.V), nn = (int)(c / d.V), f = (int)(c % d.V);
+    long k1, k2;
+    double s = sgn_pair(m, nn, k1) * sgn_pair(e, f, k2);
+    D[idx] = (s == 0.0) ? 0.0 : s * oovv[k1 * d.pv + k2];
+  }
+}
+
+__global__ void build_ovvv_u_kernel(double* U, const double* ovvv, Dims d) {
+  const long VV = (long)d.V * d.V;
+  const long n = d.ov * VV;
+  GRID_LOOP(idx, n) {
+    long row = idx / VV;  // (m,b)
+    long ef = idx % VV;
+    int e = (int)(ef / d.V), f = (int)(ef % d.V);
+    long k;
+    double s = sgn_pair(e, f, k);
+    U[idx] = (s == 0.0) ? 0.0 : s * ovvv[row * d.pv + k];
+  }
+}
+
+__global__ void build_oovv_ou_kernel(double* U, const double* oovv, Dims d) {
+  const long n = (long)d.O * d.O * d.pv;
+  GRID_LOOP(idx, n) {
+    long mn = idx / d.pv, ef = idx % d.pv;
+    int m = (int)(mn / d.O), nn = (int)(mn % d.O);
+    long k;
+    double s = sgn_pair(m, nn, k);
+    U[idx] = (s == 0.0) ? 0.0 : s * oovv[k * d.pv + ef];
+  }
+}
+
+__global__ void build_oovv_vu_kernel(double* U, const double* oovv, Dims d) {
+  const long VV = (long)d.V * d.V;
+  const long n = d.po * VV;
+  GRID_LOOP(idx, n) {
+    long mn = idx / VV, ef = idx % VV;
+    int e = (int)(ef / d.V), f = (int)(ef % d.V);
+    long k;
+    double s = sgn_pair(e, f, k);
+    U[idx] = (s == 0.0) ? 0.0 : s * oovv[mn * d.pv + k];
+  }
+}
+
+// ---------------------------------------------------------------- amplitude layouts
+__global__ void build_tau_kernel(double* tau, const double* t2, const double* t1, Dims d) {
+  const long n = d.po * d.pv;
+  GRID_LOOP(idx, n) {
+    long ij = idx / d.pv, ab = idx % d.pv;
+    int i, j, a, b;
+    pair_decode(ij, i, j);
+    pair_decode(ab, a, b);
+    tau[idx] = t2[idx] + t1[(long)i * d.V + a] * t1[(long)j * d.V + b] - t1[(long)i * d.V + b] * t1[(long)j * d.V + a];
+  }
+}
+
+__global__ void unpack_occ_kernel(double* T2u, double* tauu, const double* t2, const double* tau, Dims d) {
+  const long n = (long)d.O * d.O * d.pv;
+  GRID_LOOP(idx, n) {
+    long im = idx / d.pv, ab = idx % d.pv;
+    int i = (int)(im / d.O), m = (int)(im % d.O);
+    long k;
+    double s = sgn_pair(i, m, k);
+    double v1 = 0.0, v2 = 0.0;
+    if (s != 0.0) { v1 = s * t2[k * d.pv + ab]; v2 = s * tau[k * d.pv + ab]; }
+    T2u[idx] = v1;
+    tauu[idx] = v2;
+  }
+}
+
+__global__ void unpack_vir_kernel(double* Tv, const double* t2, Dims d) {
+  const long VV = (long)d.V * d.V;
+  const long n = d.po * VV;
+  GRID_LOOP(idx, n) {
+    long ij = idx / VV, ab = idx % VV;
+    int a = (int)(ab / d.V), b = (int)(ab % d.V);
+    long k;
+    double s = sgn_pair(a, b, k);
+    Tv[idx] = (s == 0.0) ? 0.0 : s * t2[ij * d.pv + k];
+  }
+}
+
+__global__ void unpack_ring_kernel(double* Tr, const double* t2, Dims d) {
+  const long n = d.ov * d.ov;
+  GRID_LOOP(idx, n) {
+    long r = idx / d.ov, c = idx % d.ov;  // r = (i,a), c = (m,e)
+    int i = (int)(r / d.V), a = (int)(r % d.V), m = (int)(c / d.V), e = (int)(c % d.V);
+    long k1, k2;
+    double s = sgn_pair(i, m, k1) * sgn_pair(a, e, k2);
+    Tr[idx] = (s == 0.0) ? 0.0 : s * t2[k1 * d.pv + k2];
+  }
+}
+
+// ---------------------------------------------------------------- intermediates
+__global__ void fme_init_kernel(double* Fme, const double* f, Dims d) {
+  const long N = d.O + d.V;
+  GRID_LOOP(idx, d.ov) {
+    long m = idx / d.V, e = idx % d.V;
+    Fme[idx] = f[m * N + d.O + e];
+  }
+}
+
+// W1[mn][ij] += Q[mn][i][j] - Q[mn][j][i]
+__global__ void w1_finish_kernel(double* W1, const double* Q, Dims d) {
+  const long n = d.po * d.po;
+  GRID_LOOP(idx, n) {
+    long mn = idx / d.po, ij = idx % d.po;
+    int i, j;
+    pair_decode(ij, i, j);
+    const double* q = Q + mn * d.O * d.O;
+    W1[idx] += q[(long)i * d.O + j] - q[(long)j * d.O + i];
+  }
+}
+
+// W3[i][mn][e] = ooov[mn][i*V+e]
+__global__ void w3_init_kernel(double* W3, const double* ooov, Dims d) {
+  const long n = (long)d.O * d.po * d.V;
+  GRID_LOOP(idx, n) {
+    long i = idx / (d.po * d.V);
+    long rem = idx % (d.po * d.V);
+    long mn = rem / d.V, e = rem % d.V;
+    W3[idx] = ooov[mn * d.ov + i * d.V + e];
+  }
+}
+
+// W3u[j][m][n][e] = W_mnje (antisymmetric in m,n) from W3[j][mn][e]
+__global__ void w3_unpack_kernel(double* W3u, const double* W3, Dims d) {
+  const long n = (long)d.O * d.O * d.O * d.V;
+  GRID_LOOP(idx, n) {
+    long e = idx % d.V;
+    long t = idx / d.V;
+    int nn = (int)(t % d.O);
+    t /= d.O;
+    int m = (int)(t % d.O);
+    long j = t / d.O;
+    long k;
+    double s = sgn_pair(m, nn, k);
+    W3u[idx] = (s == 0.0) ? 0.0 : s * W3[(j * d.po + k) * d.V + e];
+  }
+}
+
+// W2[ij][(m,b)] += U[i][j][(m,b)] - U[j][i][(m,b)]
+__global__ void w2_finish_kernel(double* W2, const double* U, Dims d) {
+  const long n = d.po * d.ov;
+  GRID_LOOP(idx, n) {
+    long ij = idx / d.ov, mb = idx % d.ov;
+    int i, j;
+    pair_decode(ij, i, j);
+    W2[idx] += U[((long)i * d.O + j) * d.ov + mb] - U[((long)j * d.O + i) * d.ov + mb];
+  }
+}
+
+// Fae_t1[a][e] = -sum_m WT[(m,a),(m,e)]  (WT holding sum_f t_j^f <mb||ef> only)
+__global__ void fae_t1_kernel(double* F, const double* WT, Dims d) {
+  const long n = (long)d.V * d.V;
+  GRID_LOOP(idx, n) {
+    long a = idx / d.V, e = idx % d.V;
+    double s = 0.0;
+    for (int m = 0; m < d.O; ++m) s += WT[((long)m * d.V + a) * d.ov + (long)m * d.V + e];
+    F[idx] = -s;
+  }
+}
+
+// Fmi[m][i] = (1-delta) f_mi + sum_ne t_n^e <mn||ie>
+__global__ void fmi_init_kernel(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d) {
+  const long N = d.O + d.V;
+  const long n = (long)d.O * d.O;
+  GRID_LOOP(idx, n) {
+    int m = (int)(idx / d.O), i = (int)(idx % d.O);
+    double s = (m == i) ? 0.0 : f[(long)m * N + i];
+    for (int nn = 0; nn < d.O; ++nn) {
+      long k;
+      double sg = sgn_pair(m, nn, k);
+      if (sg == 0.0) continue;
+      const double* row = ooov + k * d.ov + (long)i * d.V;
+      const double* tn = t1 + (long)nn * d.V;
+      double acc = 0.0;
+      for (int e = 0; e < d.V; ++e) acc += tn[e] * row[e];
+      s += sg * acc;
+    }
+    Fmi[idx] = s;
+  }
+}
+
+// Fmi[m][i] += sum_e Fme[m][e] t1[i][e]
+__global__ void fmi_finish_kernel(double* Fmi, const double* Fme, const double* t1, Dims d) {
+  const long n = (long)d.O * d.O;
+  GRID_LOOP(idx, n) {
+    int m = (int)(idx / d.O), i = (int)(idx % d.O);
+    double s = 0.0;
+    for (int e = 0; e < d.V; ++e) s += Fme[(long)m * d.V + e] * t1[(long)i * d.V + e];
+    Fmi[idx] += s;
+  }
+}
+
+// Fae[a][e] = (1-delta) f_ae + Fae_t1[a][e] - sum_m Fme[m][e] t1[m][a]
+__global__ void fae_init_kernel(double* Fae, const double* f, const double* Ft1, const double* Fme, const double* t1,
+                                Dims d) {
+  const long N = d.O + d.V;
+  const long n = (long)d.V * d.V;
+  GRID_LOOP(idx, n) {
+    int a = (int)(idx / d.V), e = (int)(idx % d.V);
+    double s = (a == e) ? 0.0 : f[(long)(d.O + a) * N + d.O + e];
+    s += Ft1[idx];
+    for (int m = 0; m < d.O; ++m) s -= Fme[(long)m * d.V + e] * t1[(long)m * d.V + a];
+    Fae[idx] = s;
+  }
+}
+
+// R1[i][a] = f_ia + sum_e t1[i][e] (1-delta) f_ae
+__global__ void r1_init_kernel(double* R1, const double* f, const double* t1, Dims d) {
+  const long N = d.O + d.V;
+  GRID_LOOP(idx, d.ov) {
+    int i = (int)(idx / d.V), a = (int)(idx % d.V);
+    double s = f[(long)i * N + d.O + a];
+    const double* fr = f + (long)(d.O + a) * N + d.O;
+    for (int e = 0; e < d.V; ++e)
+      if (e != a) s += t1[(long)i * d.V + e] * fr[e];
+    R1[idx] = s;
+  }
+}
+
+// ---------------------------------------------------------------- assembly / update
+__global__ void assemble_t2_kernel(double* out, double* err, const double* Rp, const double* X, const double* Y,
+                                   const double* Z, const double* f, const double* t2o, int mode, Dims d) {
+  const long n = d.po * d.pv;
+  const long N = d.O + d.V;
+  GRID_LOOP(idx, n) {
+    long ij = idx / d.pv, ab = idx % d.pv;
+    int i, j, a, b;
+    pair_decode(ij, i, j);
+    pair_decode(ab, a, b);
+    const long ia = (long)i * d.V + a, ib = (long)i * d.V + b, ja = (long)j * d.V + a, jb = (long)j * d.V + b;
+    double r = Rp[idx];
+    r += X[ia * d.ov + jb] - X[ja * d.ov + ib] - X[ib * d.ov + ja] + X[jb * d.ov + ia];
+    const double* y = Y + ij * d.V * d.V;
+    r += y[(long)a * d.V + b] - y[(long)b * d.V + a];
+    r += Z[((long)i * d.O + j) * d.pv + ab] - Z[((long)j * d.O + i) * d.pv + ab];
+    if (mode == 0) {
+      out[idx] = r;
+    } else {
+      double D = f[(long)i * N + i] + f[(long)j * N + j] - f[(long)(d.O + a) * N + d.O + a] -
+                 f[(long)(d.O + b) * N + d.O + b];
+      double t = r / D;
+      out[idx] = t;
+      err[idx] = t - t2o[idx];
+    }
+  }
+}
+
+__global__ void mp2_guess_kernel(double* amp, const double* oovv, const double* f, Dims d) {
+  const long N = d.O + d.V;
+  const long n = d.ov + d.po * d.pv;
+  GRID_LOOP(idx, n) {
+    if (idx < d.ov) {
+      amp[idx] = 0.0;
+    } else {
+      long e = idx - d.ov;
+      long ij = e / d.pv, ab = e % d.pv;
+      int i, j, a, b;
+      pair_decode(ij, i, j);
+      pair_decode(ab, a, b);
+      double D = f[(long)i * N + i] + f[(long)j * N + j] - f[(long)(d.O + a) * N + d.O + a] -
+                 f[(long)(d.O + b) * N + d.O + b];
+      amp[idx] = oovv[e] / D;
+    }
+  }
+}
+
+__global__ void update_t1_kernel(double* t1n, double* err, const double* R1, const double* f, const double* t1o,
+                                 Dims d) {
+  const long N = d.O + d.V;
+  GRID_LOOP(idx, d.ov) {
+    int i = (int)(idx / d.V), a = (int)(idx % d.V);
+    double D = f[(long)i * N + i] - f[(long)(d.O + a) * N + d.O + a];
+    double t = R1[idx] / D;
+    t1n[idx] = t;
+    err[idx] = t - t1o[idx];
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kThreads / 64; ++k) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double sh[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kThreads / 64; ++k) s = fmax(s, sh[k]);
+  __syncthreads();
+  return s;
+}
+
+__global__ void dot_partials_kernel(double* partial, const double* x, const double* y, long n) {
+  double s = 0.0;
+  GRID_LOOP(i, n) s += x[i] * y[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void maxabs_partials_kernel(double* partial, const double* x, long n) {
+  double s = 0.0;
+  GRID_LOOP(i, n) s = fmax(s, fabs(x[i]));
+  s = block_max(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void multidot_partials_kernel(double* partial, const double* X, long ldx, int nv, const double* y, long n) {
+  for (int v = 0; v < nv; ++v) {
+    const double* x = X + (long)v * ldx;
+    double s = 0.0;
+    GRID_LOOP(i, n) s += x[i] * y[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[(long)v * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ void lincomb_kernel(double* y, const double* X, long ldx, const double* c, int nv, long n) {
+  GRID_LOOP(i, n) {
+    double s = 0.0;
+    for (int v = 0; v < nv; ++v) s += c[v] * X[(long)v * ldx + i];
+    y[i] = s;
+  }
+}
+
+// out[s] = sum_p partial[s*nparts + p], fixed order
+__global__ void sum_partials_kernel(double* out, const double* partial, int nparts, int nsums) {
+  int s = blockIdx.x;
+  if (s >= nsums) return;
+  __shared__ double sh[kThreads];
+  double v = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) v += partial[(long)s * nparts + p];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = sh[0];
+}
+
+// ---------------------------------------------------------------- synthetic generator
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t hstream(uint64_t seed, uint64_t c) {
+  return mix64(seed + (c + 1ull) * 0x9E3779B97F4A7C15ull);
+}
+
+// Global orbital pair for row/col index of a block axis: kind 0 = oo pair, 1 = ov, 2 = vv pair.
+__device__ __forceinline__ void axis_orbitals(int kind, long k, const Dims& d, int& p, int& q) {
+  if (kind == 0) {
+    pair_decode(k, p, q);
+  } else if (kind == 1) {
+    p = (int)(k / d.V);
+    q = d.O + (int)(k % d.V);
+  } else {
+    pair_decode(k, p, q);
+    p += d.O;
+    q += d.O;
+  }
+}
+
+__global__ void synth_block_kernel(double* out, int rk, int ck, long r0, long r1, long ncols, Dims d, uint64_t seed,
+                                   double scale) {
+  const long n = (r1 - r0) * ncols;
+  const double two53 = 1.1102230246251565e-16;  // 2^-53
+  const double twopi = 2.0 * 3.141592653589793;
+  GRID_LOOP(idx, n) {
+    long r = r0 + idx / ncols, c = idx % ncols;
+    int p, q, rr, s;
+    axis_orbitals(rk, r, d, p, q);
+    axis_orbitals(ck, c, d, rr, s);
+    // p<q and rr<s hold by construction for every block axis.
+    uint64_t P = (uint64_t)pidx(p, q), R = (uint64_t)pidx(rr, s);
+    uint64_t lo = P < R ? P : R, hi = P < R ? R : P;
+    uint64_t K = hi * (hi + 1) / 2 + lo;
+    double u1 = ((double)(hstream(seed, 2 * K) >> 11) + 1.0) * two53;
+    double u2 = (double)(hstream(seed, 2 * K + 1) >> 11) * two53;
+    double g = scale * sqrt(-2.0 * log(u1)) * cos(twopi * u2);
+    out[idx] = g;
+  }
+}
+
+}  // namespace
+
+// ============================================================================ launchers
+#define LAUNCH(kern, n, ...) hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(kThreads), 0, s, __VA_ARGS__)
+
+void k_copy(double* dst, const double* src, long n, hipStream_t s) { LAUNCH(copy_kernel, n, dst, src, n); }
+void k_axpy(double* y, const double* x, double a, long n, hipStream_t s) { LAUNCH(axpy_kernel, n, y, x, a, n); }
+void k_fill(double* y, double v, long n, hipStream_t s) { LAUNCH(fill_kernel, n, y, v, n); }
+void k_to_f32(float* dst, const double* src, long n, hipStream_t s) { LAUNCH(to_f32_kernel, n, dst, src, n); }
+void k_split_bf16(uint16_t* hi, uint16_t* lo, const double* src, long n, hipStream_t s) {
+  LAUNCH(split_bf16_kernel, n, hi, lo, src, n);
+}
+
+void k_build_G(double* G, const double* ovov, Dims d, hipStream_t s) { LAUNCH(build_G_kernel, d.ov * d.ov, G, ovov, d); }
+void k_build_Doovv(double* D, const double* oovv, Dims d, hipStream_t s) {
+  LAUNCH(build_Doovv_kernel, d.ov * d.ov, D, oovv, d);
+}
+void k_build_ovvv_u(double* U, const double* ovvv, Dims d, hipStream_t s) {
+  LAUNCH(build_ovvv_u_kernel, d.ov * d.V * d.V, U, ovvv, d);
+}
+void k_build_oovv_ou(double* U, const double* oovv, Dims d, hipStream_t s) {
+  LAUNCH(build_oovv_ou_kernel, (long)d.O * d.O * d.pv, U, oovv, d);
+}
+void k_build_oovv_vu(double* U, const double* oovv, Dims d, hipStream_t s) {
+  LAUNCH(build_oovv_vu_kernel, d.po * d.V * d.V, U, oovv, d);
+}
+
+void k_build_tau(double* tau, const double* t2, const double* t1, Dims d, hipStream_t s) {
+  LAUNCH(build_tau_kernel, d.po * d.pv, tau, t2, t1, d);
+}
+void k_unpack_occ(double* T2u, double* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
+  LAUNCH(unpack_occ_kernel, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
+}
+void k_unpack_vir(double* Tv, const double* t2, Dims d, hipStream_t s) {
+  LAUNCH(unpack_vir_kernel, d.po * d.V * d.V, Tv, t2, d);
+}
+void k_unpack_ring(double* Tr, const double* t2, Dims d, hipStream_t s) {
+  LAUNCH(unpack_ring_kernel, d.ov * d.ov, Tr, t2, d);
+}
+
+void k_fme_init(double* Fme, const double* f, Dims d, hipStream_t s) { LAUNCH(fme_init_kernel, d.ov, Fme, f, d); }
+void k_w1_finish(double* W1, const double* Q, Dims d, hipStream_t s) { LAUNCH(w1_finish_kernel, d.po * d.po, W1, Q, d); }
+void k_w3_init(double* W3, const double* ooov, Dims d, hipStream_t s) {
+  LAUNCH(w3_init_kernel, (long)d.O * d.po * d.V, W3, ooov, d);
+}
+void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s) {
+  LAUNCH(w3_unpack_kernel, (long)d.O * d.O * d.O * d.V, W3u, W3, d);
+}
+void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s) { LAUNCH(w2_finish_kernel, d.po * d.ov, W2, U, d); }
+void k_fae_t1_extract(double* F, const double* WT, Dims d, hipStream_t s) {
+  LAUNCH(fae_t1_kernel, (long)d.V * d.V, F, WT, d);
+}
+void k_fmi_init(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d, hipStream_t s) {
+  LAUNCH(fmi_init_kernel, (long)d.O * d.O, Fmi, f, ooov, t1, d);
+}
+void k_fmi_finish(double* Fmi, const double* Fme, const double* t1, Dims d, hipStream_t s) {
+  LAUNCH(fmi_finish_kernel, (long)d.O * d.O, Fmi, Fme, t1, d);
+}
+void k_fae_init(double* Fae, const double* f, const double* Ft1, const double* Fme, const double* t1, Dims d,
+                hipStream_t s) {
+  LAUNCH(fae_init_kernel, (long)d.V * d.V, Fae, f, Ft1, Fme, t1, d);
+}
+void k_r1_init(double* R1, const double* f, const double* t1, Dims d, hipStream_t s) {
+  LAUNCH(r1_init_kernel, d.ov, R1, f, t1, d);
+}
+
+void k_assemble_t2(double* out, double* err, const double* Rp, const double* X, const double* Y, const double* Z,
+                   const double* f, const double* t2_old, int mode, Dims d, hipStream_t s) {
+  LAUNCH(assemble_t2_kernel, d.po * d.pv, out, err, Rp, X, Y, Z, f, t2_old, mode, d);
+}
+void k_mp2_guess(double* amp, const double* oovv, const double* f, Dims d, hipStream_t s) {
+  LAUNCH(mp2_guess_kernel, d.ov + d.po * d.pv, amp, oovv, f, d);
+}
+void k_update_t1(double* t1n, double* err, const double* R1, const double* f, const double* t1_old, Dims d,
+                 hipStream_t s) {
+  LAUNCH(update_t1_kernel, d.ov, t1n, err, R1, f, t1_old, d);
+}
+
+int reduce_blocks_for(long n) { return (int)std::max<long>(1, std::min<long>((n + kThreads - 1) / kThreads, 1024)); }
+
+void k_dot_partials(double* partial, const double* x, const double* y, long n, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dot_partials_kernel, dim3(nblocks), dim3(kThreads), 0, s, partial, x, y, n);
+}
+void k_maxabs_partials(double* partial, const double* x, long n, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(maxabs_partials_kernel, dim3(nblocks), dim3(kThreads), 0, s, partial, x, n);
+}
+void k_multidot_partials(double* partial, const double* X, long ldx, int nv, const double* y, long n, int nblocks,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(multidot_partials_kernel, dim3(nblocks), dim3(kThreads), 0, s, partial, X, ldx, nv, y, n);
+}
+void k_lincomb(double* y, const double* X, long ldx, const double* c, int nv, long n, hipStream_t s) {
+  LAUNCH(lincomb_kernel, n, y, X, ldx, c, nv, n);
+}
+void k_sum_partials(double* out, const double* partial, int nparts, int nsums, hipStream_t s) {
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(nsums), dim3(kThreads), 0, s, out, partial, nparts, nsums);
+}
+
+void k_synth_block(double* out, int block, long r0, long r1, Dims d, uint64_t seed, double scale, hipStream_t s) {
+  static const int rk[6] = {0, 0, 0, 1, 1, 2};
+  static const int ck[6] = {0, 1, 2, 1, 2, 2};
+  long ncols = (ck[block] == 0) ? d.po : (ck[block] == 1 ? d.ov : d.pv);
+  long n = (r1 - r0) * ncols;
+  LAUNCH(synth_block_kernel, n, out, rk[block], ck[block], r0, r1, ncols, d, seed, scale);
+}
+
+}  // namespace ccsd

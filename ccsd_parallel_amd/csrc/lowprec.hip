@@ -1,0 +1,433 @@
+// Mixed-precision GEMM path: FP32 MFMA (v_mfma_f32_16x16x4f32 through the
+// generic engine) and BF16 hi/lo split (v_mfma_f32_16x16x32_bf16, three
+// products hi*hi + hi*lo + lo*hi accumulated in FP32).
+//
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi) keeps ~16 significant bits
+// of each operand; the dropped lo*lo term and FP32 accumulation put the
+// product error near 2^-16 relative, which the final FP64 sweeps remove.
+#include <algorithm>
+#include <cstring>
+
+#include "lowprec.hpp"
+
+namespace ccsd {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct LowPrec {
+  int have_fp32 = 0, have_bf16 = 0;
+  long ld_pv = 0, ld_ov = 0;  // padded leading dims for K = pv and K = ov
+  // FP32 operands (k-contiguous, padded)
+  float *vvvv32 = nullptr, *ovvv32 = nullptr, *Doovv32 = nullptr;
+  float *tau32 = nullptr, *Tring32 = nullptr, *WT32 = nullptr;
+  // BF16 hi/lo operands
+  uint16_t *vvvv_h = nullptr, *vvvv_l = nullptr, *ovvv_h = nullptr, *ovvv_l = nullptr;
+  uint16_t *Doovv_h = nullptr, *Doovv_l = nullptr;
+  uint16_t *tau_h = nullptr, *tau_l = nullptr, *Tring_h = nullptr, *Tring_l = nullptr, *WT_h = nullptr,
+           *WT_l = nullptr;
+  float* C32 = nullptr;  // FP32 product scratch
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+constexpr int kThreads = 256;
+inline long pad64(long k) { return ((k + 63) / 64) * 64; }
+inline int grid_for(long n) {
+  long b = (n + kThreads - 1) / kThreads;
+  return (int)std::max<long>(1, std::min<long>(b, 1L << 20));
+}
+#define GRID_LOOP(idx, n) \
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (n); idx += (long)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__global__ void conv_f32_kernel(float* dst, const double* src, long rows, long K, long sld, long ldk, double scale) {
+  const long n = rows * ldk;
+  GRID_LOOP(idx, n) {
+    long r = idx / ldk, k = idx % ldk;
+    dst[idx] = (k < K) ? (float)(scale * src[r * sld + k]) : 0.0f;
+  }
+}
+
+__global__ void conv_bf16x2_kernel(uint16_t* hi, uint16_t* lo, const double* src, long rows, long K, long sld,
+                                   long ldk, double scale) {
+  const long n = rows * ldk;
+  GRID_LOOP(idx, n) {
+    long r = idx / ldk, k = idx % ldk;
+    uint16_t h = 0, l = 0;
+    if (k < K) {
+      double x = scale * src[r * sld + k];
+      h = bf16_rne((float)x);
+      l = bf16_rne((float)(x - (double)bf16_to_f(h)));
+    }
+    hi[idx] = h;
+    lo[idx] = l;
+  }
+}
+
+// y[r*ldy + c] = alpha * x[r*ldx + c] + beta * y  (FP32 product into FP64 destination)
+__global__ void add_f32_kernel(double* y, long ldy, const float* x, long ldx, long rows, long cols, double alpha,
+                               double beta) {
+  const long n = rows * cols;
+  GRID_LOOP(idx, n) {
+    long r = idx / cols, c = idx % cols;
+    double v = alpha * (double)x[r * ldx + c];
+    if (beta != 0.0) v += beta * y[r * ldy + c];
+    y[r * ldy + c] = v;
+  }
+}
+
+struct Bf16Args {
+  const uint16_t *Ahi, *Alo, *Bhi, *Blo;
+  float* C;
+  long lda, ldb, ldc;
+  int M, N, Kp;  // Kp: K rounded up to 32 (operands zero padded)
+  int tiles_m, tiles_n, tiles_pad;
+};
+
+// C[m][n] = sum_k A[m][k] B[n][k] with A = Ahi + Alo, B = Bhi + Blo, dropping lo*lo.
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BK = 32;
+  constexpr int LDK = BK + 8;  // 80-byte row pitch: 16 lanes of one k-slice hit distinct banks
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int CA = 2 * BM * 4 / NT;  // 16-byte chunks per thread (hi+lo)
+  constexpr int CB = 2 * BN * 4 / NT;
+  static_assert(CA * NT == 2 * BM * 4 && CB * NT == 2 * BN * 4, "tile/threads mismatch");
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * LDK];
+
+  const int xcd = blockIdx.x & 7;
+  const int tile = xcd * (p.tiles_pad >> 3) + (blockIdx.x >> 3);
+  if (tile >= p.tiles_m * p.tiles_n) return;
+  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
+
+  uint4 ra[CA], rb[CB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < CA; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
+      const uint16_t* src = (hl ? p.Alo : p.Ahi) + (long)(m0 + m) * p.lda + k0 + ch * 8;
+      ra[it] = (m0 + m < p.M) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int it = 0; it < CB; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
+      const uint16_t* src = (hl ? p.Blo : p.Bhi) + (long)(n0 + n) * p.ldb + k0 + ch * 8;
+      rb[it] = (n0 + n < p.N) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int it = 0; it < CA; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
+      *(uint4*)&sA[hl][m * LDK + ch * 8] = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < CB; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
+      *(uint4*)&sB[hl][n * LDK + ch * 8] = rb[it];
+    }
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+  const int nk = p.Kp / BK;
+  if (nk > 0) load(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    __syncthreads();  // previous compute finished reading LDS
+    store();
+    __syncthreads();
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    const int kofs = 8 * (lane >> 4);
+    bf16x8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = (wm0 + i * 16 + (lane & 15)) * LDK + kofs;
+      ah[i] = *(const bf16x8_t*)&sA[0][r];
+      al[i] = *(const bf16x8_t*)&sA[1][r];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = (wn0 + j * 16 + (lane & 15)) * LDK + kofs;
+      bh[j] = *(const bf16x8_t*)&sB[0][r];
+      bl[j] = *(const bf16x8_t*)&sB[1][r];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (row < p.M && col < p.N) p.C[(long)row * p.ldc + col] = acc[i][j][r];
+      }
+}
+
+hipError_t bf16x3_nt(const uint16_t* Ahi, const uint16_t* Alo, long lda, const uint16_t* Bhi, const uint16_t* Blo,
+                     long ldb, float* C, long ldc, int M, int N, int K, hipStream_t s) {
+  Bf16Args p;
+  p.Ahi = Ahi; p.Alo = Alo; p.Bhi = Bhi; p.Blo = Blo; p.C = C;
+  p.lda = lda; p.ldb = ldb; p.ldc = ldc;
+  p.M = M; p.N = N; p.Kp = ((K + 31) / 32) * 32;
+  constexpr int BM = 128, BN = 128;
+  p.tiles_m = (M + BM - 1) / BM;
+  p.tiles_n = (N + BN - 1) / BN;
+  long tiles = (long)p.tiles_m * p.tiles_n;
+  p.tiles_pad = (int)(((tiles + 7) / 8) * 8);
+  hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, 2, 2>), dim3(p.tiles_pad), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t lowprec_plan_bytes(const Dims& d, int precision) {
+  if (precision == CCSD_PREC_FP64) return 0;
+  const long lpv = pad64(d.pv), lov = pad64(d.ov);
+  long elems = d.pv * lpv + d.ov * lpv + d.ov * lov + d.po * lpv + 2 * d.ov * lov;
+  size_t per = (precision == CCSD_PREC_FP32) ? 4 : 4;  // fp32: 4 B; bf16 hi+lo: 2 x 2 B
+  long c32 = std::max(std::max(d.po * d.pv, d.po * d.ov), d.ov * d.ov);
+  return (size_t)elems * per + (size_t)c32 * 4;
+}
+
+template <typename T>
+static int lp_alloc(Ctx* c, T** p, long n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, (size_t)std::max<long>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  c->lp->allocs.push_back(q);
+  c->bytes += (size_t)std::max<long>(n, 1) * sizeof(T);
+  *p = (T*)q;
+  return CCSD_OK;
+}
+
+#define TRY(x)                      \
+  do {                              \
+    int _rc = (x);                  \
+    if (_rc != CCSD_OK) return _rc; \
+  } while (0)
+
+int lowprec_allocate(Ctx* c, int precision) {
+  if (precision == CCSD_PREC_FP64) return CCSD_OK;
+  if (precision != CCSD_PREC_FP32 && precision != CCSD_PREC_BF16X3) return CCSD_ERR_ARG;
+  if (!c->lp) c->lp = new LowPrec();
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  L->ld_pv = pad64(d.pv);
+  L->ld_ov = pad64(d.ov);
+  hipStream_t s = c->stream;
+  if (!L->C32) TRY(lp_alloc(c, &L->C32, std::max(std::max(d.po * d.pv, d.po * d.ov), d.ov * d.ov)));
+  if (precision == CCSD_PREC_FP32 && !L->have_fp32) {
+    TRY(lp_alloc(c, &L->vvvv32, d.pv * L->ld_pv));
+    TRY(lp_alloc(c, &L->ovvv32, d.ov * L->ld_pv));
+    TRY(lp_alloc(c, &L->Doovv32, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->tau32, d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->Tring32, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->WT32, d.ov * L->ld_ov));
+    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.pv * L->ld_pv)), dim3(kThreads), 0, s, L->vvvv32, c->vvvv,
+                       d.pv, d.pv, d.pv, L->ld_pv, 1.0);
+    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.ov * L->ld_pv)), dim3(kThreads), 0, s, L->ovvv32, c->ovvv,
+                       d.ov, d.pv, d.pv, L->ld_pv, 1.0);
+    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, s, L->Doovv32, c->Doovv,
+                       d.ov, d.ov, d.ov, L->ld_ov, 1.0);
+    L->have_fp32 = 1;
+  }
+  if (precision == CCSD_PREC_BF16X3 && !L->have_bf16) {
+    TRY(lp_alloc(c, &L->vvvv_h, d.pv * L->ld_pv));
+    TRY(lp_alloc(c, &L->vvvv_l, d.pv * L->ld_pv));
+    TRY(lp_alloc(c, &L->ovvv_h, d.ov * L->ld_pv));
+    TRY(lp_alloc(c, &L->ovvv_l, d.ov * L->ld_pv));
+    TRY(lp_alloc(c, &L->Doovv_h, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->Doovv_l, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->tau_h, d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->tau_l, d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->Tring_h, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->Tring_l, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->WT_h, d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->WT_l, d.ov * L->ld_ov));
+    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.pv * L->ld_pv)), dim3(kThreads), 0, s, L->vvvv_h,
+                       L->vvvv_l, c->vvvv, d.pv, d.pv, d.pv, L->ld_pv, 1.0);
+    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.ov * L->ld_pv)), dim3(kThreads), 0, s, L->ovvv_h,
+                       L->ovvv_l, c->ovvv, d.ov, d.pv, d.pv, L->ld_pv, 1.0);
+    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, s, L->Doovv_h,
+                       L->Doovv_l, c->Doovv, d.ov, d.ov, d.ov, L->ld_ov, 1.0);
+    L->have_bf16 = 1;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  return CCSD_OK;
+}
+
+void lowprec_free(Ctx* c) {
+  if (!c->lp) return;
+  for (void* p : c->lp->allocs) (void)hipFree(p);
+  delete c->lp;
+  c->lp = nullptr;
+}
+
+static void conv(Ctx* c, int prec, const double* src, long rows, long K, long ld, float* d32, uint16_t* dh,
+                 uint16_t* dl) {
+  long n = rows * ld;
+  if (prec == CCSD_PREC_FP32)
+    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, d32, src, rows, K, K, ld,
+                       1.0);
+  else
+    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, dh, dl, src, rows, K,
+                       K, ld, 1.0);
+}
+
+int lowprec_amplitudes(Ctx* c, int prec) {
+  TRY(lowprec_allocate(c, prec));
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  conv(c, prec, c->tau, d.po, d.pv, L->ld_pv, L->tau32, L->tau_h, L->tau_l);
+  conv(c, prec, c->Tring, d.ov, d.ov, L->ld_ov, L->Tring32, L->Tring_h, L->Tring_l);
+  return CCSD_OK;
+}
+
+// C64 (M x N, ldc) = alpha * A B^T + beta * C64, operands as stored low-precision pairs.
+static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ah, const uint16_t* Al, long lda,
+                      const float* B32, const uint16_t* Bh, const uint16_t* Bl, long ldb, int M, int N, int K,
+                      double* C, long ldc, double alpha, double beta) {
+  LowPrec* L = c->lp;
+  hipError_t e;
+  if (prec == CCSD_PREC_FP32) {
+    GemmDesc<float> g;
+    g.A = A32; g.B = B32; g.C = L->C32;
+    g.M = M; g.N = N; g.K = K;
+    g.sAm = lda; g.sAk = 1; g.sBk = 1; g.sBn = ldb; g.sCm = N; g.sCn = 1;
+    g.alpha = 1.0f; g.beta = 0.0f;
+    GemmWorkspace ws = c->ws;
+    e = gemm<float>(g, ws, c->stream);
+  } else {
+    e = bf16x3_nt(Ah, Al, lda, Bh, Bl, ldb, L->C32, N, M, N, K, c->stream);
+  }
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  hipLaunchKernelGGL(add_f32_kernel, dim3(grid_for((long)M * N)), dim3(kThreads), 0, c->stream, C, ldc,
+                     (const float*)L->C32, (long)N, (long)M, (long)N, alpha, beta);
+  return CCSD_OK;
+}
+
+int lowprec_w2(Ctx* c, int prec) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  return nt_product(c, prec, L->tau32, L->tau_h, L->tau_l, L->ld_pv, L->ovvv32, L->ovvv_h, L->ovvv_l, L->ld_pv,
+                    (int)d.po, (int)d.ov, (int)d.pv, c->W2, d.ov, 1.0, 1.0);
+}
+
+int lowprec_wmbej(Ctx* c, int prec) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  // Doovv is symmetric, so its rows serve as the k-contiguous B operand.
+  return nt_product(c, prec, L->Tring32, L->Tring_h, L->Tring_l, L->ld_ov, L->Doovv32, L->Doovv_h, L->Doovv_l,
+                    L->ld_ov, (int)d.ov, (int)d.ov, (int)d.ov, c->WT, d.ov, 0.5, 1.0);
+}
+
+int lowprec_ladder(Ctx* c, int prec) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  // <ef||ab> = <ab||ef>: the packed vvvv block is symmetric, rows are k-contiguous B.
+  return nt_product(c, prec, L->tau32, L->tau_h, L->tau_l, L->ld_pv, L->vvvv32, L->vvvv_h, L->vvvv_l, L->ld_pv,
+                    (int)d.po, (int)d.pv, (int)d.pv, c->Rp, d.pv, 1.0, 1.0);
+}
+
+int lowprec_ring(Ctx* c, int prec) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  conv(c, prec, c->WT, d.ov, d.ov, L->ld_ov, L->WT32, L->WT_h, L->WT_l);
+  return nt_product(c, prec, L->Tring32, L->Tring_h, L->Tring_l, L->ld_ov, L->WT32, L->WT_h, L->WT_l, L->ld_ov,
+                    (int)d.ov, (int)d.ov, (int)d.ov, c->X, d.ov, 1.0, 0.0);
+}
+
+int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K, double alpha, double beta,
+                    int prec, hipStream_t s) {
+  const long ld = pad64(K);
+  float *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
+  uint16_t *Ah = nullptr, *Al = nullptr, *Bh = nullptr, *Bl = nullptr;
+  std::vector<void*> owned;
+  auto al = [&](void** p, size_t b) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(b, 16));
+    if (e == hipSuccess) owned.push_back(*p);
+    return e;
+  };
+  hipError_t e = al((void**)&C32, (size_t)M * N * 4);
+  if (prec == CCSD_PREC_FP32) {
+    if (e == hipSuccess) e = al((void**)&A32, (size_t)M * ld * 4);
+    if (e == hipSuccess) e = al((void**)&B32, (size_t)N * ld * 4);
+  } else {
+    if (e == hipSuccess) e = al((void**)&Ah, (size_t)M * ld * 2);
+    if (e == hipSuccess) e = al((void**)&Al, (size_t)M * ld * 2);
+    if (e == hipSuccess) e = al((void**)&Bh, (size_t)N * ld * 2);
+    if (e == hipSuccess) e = al((void**)&Bl, (size_t)N * ld * 2);
+  }
+  void* wsp = nullptr;
+  if (e == hipSuccess) e = al(&wsp, 16u << 20);
+  if (e == hipSuccess) {
+    if (prec == CCSD_PREC_FP32) {
+      hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(M * ld)), dim3(kThreads), 0, s, A32, A, (long)M, (long)K,
+                         (long)K, ld, 1.0);
+      hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(N * ld)), dim3(kThreads), 0, s, B32, B, (long)N, (long)K,
+                         (long)K, ld, 1.0);
+      GemmDesc<float> g;
+      g.A = A32; g.B = B32; g.C = C32;
+      g.M = M; g.N = N; g.K = K;
+      g.sAm = ld; g.sAk = 1; g.sBk = 1; g.sBn = ld; g.sCm = N; g.sCn = 1;
+      GemmWorkspace ws{wsp, 16u << 20};
+      e = gemm<float>(g, ws, s);
+    } else {
+      hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(M * ld)), dim3(kThreads), 0, s, Ah, Al, A, (long)M,
+                         (long)K, (long)K, ld, 1.0);
+      hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(N * ld)), dim3(kThreads), 0, s, Bh, Bl, B, (long)N,
+                         (long)K, (long)K, ld, 1.0);
+      e = bf16x3_nt(Ah, Al, ld, Bh, Bl, ld, C32, N, M, N, K, s);
+    }
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(add_f32_kernel, dim3(grid_for((long)M * N)), dim3(kThreads), 0, s, C, (long)N,
+                       (const float*)C32, (long)N, (long)M, (long)N, alpha, beta);
+    e = hipStreamSynchronize(s);
+  }
+  for (void* p : owned) (void)hipFree(p);
+  return e == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+}
+
+}  // namespace ccsd

@@ -1,0 +1,171 @@
+/* C ABI of the MI355X CCSD amplitude-update engine.
+ *
+ * The reference has no FFI: its boundary is the C++ class
+ * CCSD_Memory_optimized (ccsd_memory_optimized.h:15-149) whose entry point
+ * `double CCSD(double toler, bool print_stuff)` (ccsd_memory_optimized.cpp:16-567)
+ * runs the whole amplitude iteration.  This header is the thin C layer that the
+ * drop-in class (ccsd_memory_optimized.h in this directory) and any other host
+ * binding call.  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * Conventions
+ *   O = occupied spin orbitals, V = virtual spin orbitals, N = O + V.
+ *   Pair index for p<q: pidx(p,q) = q*(q-1)/2 + p; P2(n) = n*(n-1)/2.
+ *   Integral blocks are antisymmetrised <pq||rs>, packed row-major:
+ *     oooo [P2(O)][P2(O)]   <ij||kl>, i<j, k<l
+ *     ooov [P2(O)][O*V]     <ij||ka>, i<j, column k*V+a
+ *     oovv [P2(O)][P2(V)]   <ij||ab>, i<j, a<b
+ *     ovov [O*V][O*V]       <ia||jb>, row i*V+a
+ *     ovvv [O*V][P2(V)]     <ia||bc>, b<c
+ *     vvvv [P2(V)][P2(V)]   <ab||cd>, a<b, c<d
+ *   fock is the full N x N Fock matrix (row-major, occupied first).
+ *   Amplitudes: t1[i*V+a]; t2 packed [P2(O)][P2(V)] holding t_ij^ab, i<j, a<b.
+ *   Return value 0 = success, negative = error (ccsd_strerror).  Host buffers
+ *   are caller-owned and copied; the context owns all device memory.
+ *   One context per thread; calls on one context are serialised on its stream.
+ */
+#ifndef CCSD_AMD_H
+#define CCSD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccsd_ctx ccsd_ctx;
+
+enum ccsd_status {
+  CCSD_OK = 0,
+  CCSD_ERR_ARG = -1,           /* null pointer, bad O/V, bad option */
+  CCSD_ERR_HIP = -2,           /* HIP runtime error */
+  CCSD_ERR_BUDGET = -3,        /* the plan exceeds the HBM budget (checked before allocating) */
+  CCSD_ERR_NONFINITE = -4,     /* NaN/Inf in inputs or in the iteration */
+  CCSD_ERR_NOT_CONVERGED = -5, /* max_iter reached; result is filled */
+  CCSD_ERR_DENOMINATOR = -6,   /* |f_ii - f_aa| below 1e-10 */
+  CCSD_ERR_NO_DEVICE = -7,     /* no HIP device */
+  CCSD_ERR_STATE = -8          /* call order (e.g. backward before solve) */
+};
+
+enum ccsd_precision {
+  CCSD_PREC_FP64 = 0,   /* all contractions in FP64 (reference mode) */
+  CCSD_PREC_FP32 = 1,   /* ladder/ring/W_mbej/W_mbij GEMMs in FP32 MFMA */
+  CCSD_PREC_BF16X3 = 2  /* same GEMMs as BF16 hi/lo split, 3 products, FP32 accumulate */
+};
+
+typedef struct ccsd_options {
+  int precision;            /* ccsd_precision used before the switch to FP64 */
+  double switch_threshold;  /* mixed modes: switch to FP64 once |dE| < this (default 1e-6) */
+  int final_fp64_sweeps;    /* minimum FP64 iterations after the switch (default 3) */
+  int diis_dim;             /* 0 or 1: plain Jacobi as the reference; >= 2: DIIS subspace (default 8) */
+  int diis_start;           /* first iteration allowed to extrapolate (default 1) */
+  int recompute_v4;         /* backward: recompute ladder intermediates rather than store (default 1) */
+  int verbose;              /* print "Energi: E Steg: n Time/iter: s" lines like the reference */
+  int device;               /* HIP device ordinal */
+  size_t hbm_budget_bytes;  /* 0 = 90 % of the device's free memory */
+} ccsd_options;
+
+typedef struct ccsd_problem {
+  int n_occ;   /* O */
+  int n_virt;  /* V */
+  const double* fock;
+  const double* oooo;
+  const double* ooov;
+  const double* oovv;
+  const double* ovov;
+  const double* ovvv;
+  const double* vvvv;
+} ccsd_problem;
+
+typedef struct ccsd_result {
+  double e_corr;          /* CCSD correlation energy */
+  double e_mp2;           /* energy of the MP2 starting guess */
+  double last_delta_e;    /* |E_n - E_{n-1}| at exit */
+  double max_delta_t2;    /* max |t2_n - t2_{n-1}| at exit */
+  int iterations;         /* amplitude updates performed */
+  int converged;
+  int diis_fallbacks;     /* DIIS solves rejected (plain Jacobi step taken) */
+  int lowp_iterations;    /* iterations run in FP32 / BF16x3 */
+  int fp64_iterations;    /* iterations run in FP64 */
+  double seconds_total;   /* wall time of the iteration loop */
+  double seconds_per_iter;
+} ccsd_result;
+
+typedef struct ccsd_grad_info {
+  int lambda_iterations;
+  int converged;
+  double lambda_residual;  /* max |Lambda_n - Lambda_{n-1}| at exit */
+  double seconds;
+} ccsd_grad_info;
+
+/* Fill *o with defaults. */
+int ccsd_default_options(ccsd_options* o);
+
+/* Create a context from host buffers (copied into HBM). */
+int ccsd_create(const ccsd_problem* p, const ccsd_options* o, ccsd_ctx** out);
+
+/* Create a context whose integrals are generated in HBM by the counter-based
+ * synthetic generator (SplitMix64 + Box-Muller, scale = standard deviation). */
+int ccsd_create_synthetic(int n_occ, int n_virt, uint64_t seed, double scale, const ccsd_options* o,
+                          ccsd_ctx** out);
+
+/* Iterate from the MP2 guess (or from amplitudes set by ccsd_set_amplitudes)
+ * until |dE| < tol or max_iter updates.  Returns CCSD_ERR_NOT_CONVERGED (and a
+ * filled result) when max_iter is reached. */
+int ccsd_solve(ccsd_ctx* c, double tol, int max_iter, ccsd_result* out);
+
+/* Run exactly n amplitude updates in the given precision without a
+ * convergence test (benchmarking); energies[k] receives E after update k. */
+int ccsd_iterate(ccsd_ctx* c, int n, int precision, double* energies);
+
+/* Reset amplitudes to the MP2 guess and clear the DIIS history. */
+int ccsd_reset(ccsd_ctx* c);
+
+/* Right-hand sides R1 (O*V) and packed R2 at the given amplitudes (FP64),
+ * i.e. D*t = R without the division. */
+int ccsd_residual(ccsd_ctx* c, const double* t1, const double* t2, double* r1, double* r2);
+
+/* Energy E = f_ia t_i^a + sum_{i<j,a<b} <ij||ab> tau_ij^ab at given amplitudes. */
+int ccsd_energy(ccsd_ctx* c, const double* t1, const double* t2, double* e);
+
+int ccsd_get_amplitudes(ccsd_ctx* c, double* t1, double* t2);
+int ccsd_set_amplitudes(ccsd_ctx* c, const double* t1, const double* t2);
+
+/* Energy of every iteration of the last solve/iterate (E of the MP2 guess
+ * first).  Returns the count; copies min(count, cap) values. */
+int ccsd_energy_trace(ccsd_ctx* c, double* out, int cap);
+
+/* Adjoint of the converged energy: solves the CCSD Lambda equations and
+ * contracts the unrelaxed densities into dE/df (N*N, symmetrised) and
+ * dE/d<pq||rs> in the packed block layout of ccsd_problem (symmetrised over
+ * duplicate entries of oooo/ovov/vvvv).  Requires a converged solve. */
+int ccsd_backward(ccsd_ctx* c, double* dE_dfock, double* dE_doooo, double* dE_dooov, double* dE_doovv,
+                  double* dE_dovov, double* dE_dovvv, double* dE_dvvvv, ccsd_grad_info* info);
+
+/* Device bytes the context holds (or would hold for O, V before create). */
+size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision);
+
+/* Per-kernel-class timing of the last iteration (milliseconds), HIP events on
+ * the context's stream: [0] pp ladder, [1] ring, [2] W_mbej quadratic,
+ * [3] W_mbij, [4] everything else.  Returns the count written. */
+int ccsd_last_timings(ccsd_ctx* c, double* ms, int cap);
+
+void ccsd_destroy(ccsd_ctx* c);
+const char* ccsd_strerror(int code);
+
+/* Test hook: copy the named FP64 intermediate of the last right-hand side
+ * ("Fme", "Fmi", "Fae", "W1", "W2", "W3", "WT", "X", "Y", "Z", "Rp", "R1",
+ * "tau", "Tring", "Tv", "T2u") into out; returns its element count or < 0. */
+long ccsd_debug_get(ccsd_ctx* c, const char* name, double* out, long cap);
+
+/* Test hook: run the MFMA GEMM engine on host matrices
+ * C(MxN) = alpha*A(MxK)*B(KxN) + beta*C, A/B/C row-major, op flags transpose
+ * the stored A (ta) / B (tb); precision selects FP64 / FP32 / BF16X3 arithmetic. */
+int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const double* A, const double* B,
+                       double beta, double* C, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCSD_AMD_H */

@@ -1,0 +1,174 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Bars: FP64 residuals and intermediates within 1e-12 of the largest entry;
+per-iteration energies within 1e-12 Eh; converged energies within 1e-10 Eh
+(the 2-electron FCI check within 1e-10 Eh).  Mixed precision: FP32 and BF16x3
+runs switch to FP64 and must land within 1e-7 Eh of the FP64 energy.
+"""
+import numpy as np
+import pytest
+
+from oracle import ccsd_ref, fci2, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx():
+    from ccsd_parallel_amd import capi
+    return capi
+
+
+def norm_scale(O, V):
+    return 1.0 / np.sqrt(O * synth.p2(V))
+
+
+def random_amplitudes(O, V, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    t1 = amp * rng.standard_normal((O, V))
+    t2 = amp * rng.standard_normal((O, O, V, V))
+    t2 = t2 - t2.transpose(1, 0, 2, 3)
+    t2 = t2 - t2.transpose(0, 1, 3, 2)
+    return t1, t2
+
+
+def packed(t2, O, V):
+    return synth.pack_pairs(synth.pack_pairs(t2, O, 0), V, 1)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [
+    (37, 53, 29, 0, 0), (128, 128, 64, 0, 1), (64, 300, 17, 1, 0), (200, 33, 1000, 1, 1), (1, 77, 500, 0, 0),
+    (190, 257, 4000, 0, 1), (5, 5, 20000, 0, 0), (300, 1, 300, 0, 0)])
+def test_gemm_fp64(M, N, K, ta, tb):
+    capi = _ctx()
+    rng = np.random.default_rng(M * 7 + N)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 0.7 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.3 * C0
+    out = capi.selftest_gemm(A, B, C0, 0.7, -0.3, ta=bool(ta), tb=bool(tb), precision="fp64")
+    assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 5e-5)])
+@pytest.mark.parametrize("M,N,K", [(130, 70, 300), (256, 256, 1024), (33, 129, 65)])
+def test_gemm_lowprec(prec, tol, M, N, K):
+    capi = _ctx()
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = A @ B + 0.5 * C0
+    out = capi.selftest_gemm(A, B, C0, 1.0, 0.5, precision=prec)
+    # error relative to the natural scale sqrt(K) of each entry
+    assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K)
+
+
+@pytest.mark.parametrize("O,V,seed", [(4, 8, 11), (6, 10, 12), (8, 20, 13), (10, 26, 14)])
+def test_residual_parity(O, V, seed):
+    capi = _ctx()
+    prob = synth.make_problem(O, V, seed, scale=0.05)
+    rng = np.random.default_rng(seed)
+    f = prob["f"] + 0.01 * (lambda a: a + a.T)(rng.standard_normal(prob["f"].shape))  # non-canonical Fock
+    prob = dict(prob, f=f)
+    I = ccsd_ref.Ints.from_problem(prob)
+    t1, t2 = random_amplitudes(O, V, seed)
+    R1, R2, _, _, X = ccsd_ref.rhs(I, t1, t2, parts=True)
+    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+        r1, r2 = ctx.residual(t1, packed(t2, O, V))
+        scale = max(np.abs(R2).max(), np.abs(R1).max())
+        # intermediates first, so a failure names the term
+        checks = {
+            "Fme": (ctx.debug("Fme"), X["Fme"].ravel()),
+            "Fmi": (ctx.debug("Fmi"), X["Fmi"].ravel()),
+            "Fae": (ctx.debug("Fae"), X["Fae"].ravel()),
+            "W1": (ctx.debug("W1"), synth.pack_pairs(synth.pack_pairs(X["W1"], O, 0), O, 1).ravel()),
+            "W2": (ctx.debug("W2"), synth.pack_pairs(X["W2"], O, 2).transpose(2, 0, 1).ravel()),
+            "W3": (ctx.debug("W3"), synth.pack_pairs(X["W3"], O, 0).transpose(1, 0, 2).ravel()),
+            "WT": (ctx.debug("WT"), X["W4"].transpose(3, 1, 0, 2).ravel()),
+        }
+        for name, (dev, ref) in checks.items():
+            err = np.max(np.abs(dev - ref))
+            assert err <= 1e-12 * max(1.0, np.abs(ref).max()), f"{name}: {err}"
+        assert np.max(np.abs(r1 - R1)) <= 1e-12 * scale
+        assert np.max(np.abs(r2 - packed(R2, O, V))) <= 1e-12 * scale
+        assert abs(ctx.energy(t1, packed(t2, O, V)) - ccsd_ref.energy(I, t1, t2)) <= 1e-13 * max(1, abs(ccsd_ref.energy(I, t1, t2)))
+
+
+def test_synthetic_generator_matches_host():
+    """Device-generated integrals equal the host generator (up to libm rounding)."""
+    capi = _ctx()
+    O, V, seed = 6, 14, 2024
+    sc = norm_scale(O, V)
+    prob = synth.make_problem(O, V, seed, scale=sc)
+    t1, t2 = random_amplitudes(O, V, 5, amp=0.02)
+    with capi.Context.from_problem(prob) as a, capi.Context.synthetic(O, V, seed, sc) as b:
+        ra = a.residual(t1, packed(t2, O, V))
+        rb = b.residual(t1, packed(t2, O, V))
+        for x, y in zip(ra, rb):
+            assert np.max(np.abs(x - y)) <= 1e-14 * max(1.0, np.abs(x).max())
+
+
+def test_energy_trace_parity_jacobi():
+    """Plain Jacobi (the reference's update, no DIIS): 40 updates, energy by energy."""
+    capi = _ctx()
+    O, V, seed = 8, 24, 20180
+    prob = synth.make_problem(O, V, seed, scale=norm_scale(O, V))
+    I = ccsd_ref.Ints.from_problem(prob)
+    E, t1, t2, info = ccsd_ref.solve(I, toler=0.0, max_iter=41, diis_dim=0)
+    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+        r = ctx.solve(0.0, 40, allow_unconverged=True)
+        tr = ctx.energy_trace()
+        g1, g2 = ctx.get_amplitudes()
+    ref = np.array(info["energies"])
+    assert r["iterations"] == 40 and len(tr) == 41 and len(ref) == 41
+    assert np.max(np.abs(tr - ref)) <= 1e-11
+    assert abs(r["e_mp2"] - ref[0]) <= 1e-13
+    assert np.max(np.abs(g2 - packed(t2, O, V))) <= 1e-10
+    assert np.max(np.abs(g1 - t1)) <= 1e-10
+
+
+def test_energy_trace_parity_diis():
+    capi = _ctx()
+    O, V, seed = 8, 24, 20180
+    prob = synth.make_problem(O, V, seed, scale=norm_scale(O, V))
+    I = ccsd_ref.Ints.from_problem(prob)
+    E, t1, t2, info = ccsd_ref.solve(I, toler=1e-10, max_iter=200, diis_dim=8)
+    with capi.Context.from_problem(prob, diis_dim=8) as ctx:
+        r = ctx.solve(1e-10, 200)
+        tr = ctx.energy_trace()
+        g1, g2 = ctx.get_amplitudes()
+    ref = np.array(info["energies"])
+    assert r["converged"] == 1 and info["converged"]
+    n = min(len(ref), len(tr))
+    assert np.max(np.abs(tr[:n] - ref[:n])) <= 1e-11
+    assert abs(r["e_corr"] - E) <= 1e-10
+    assert np.max(np.abs(g2 - packed(t2, O, V))) <= 1e-8
+
+
+def test_two_electron_fci():
+    capi = _ctx()
+    O, V, seed = 2, 10, 77
+    g = synth.full_tensor(O, V, seed, scale=0.05)
+    eo, ev = synth.orbital_energies(O, V, seed)
+    f = np.diag(np.concatenate([eo, ev]))
+    prob = dict(O=O, V=V, f=f, **synth.pack_dense(synth.blocks_from_full(g, O), O, V))
+    e_fci = fci2.fci_two_electron(f, g)[0]
+    with capi.Context.from_problem(prob) as ctx:
+        r = ctx.solve(1e-12, 200)
+    assert abs(r["e_corr"] - e_fci) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_converges_to_fp64(prec):
+    capi = _ctx()
+    O, V, seed = 10, 40, 4321
+    sc = norm_scale(O, V)
+    with capi.Context.synthetic(O, V, seed, sc) as ctx:
+        r64 = ctx.solve(1e-10, 200)
+        t1a, t2a = ctx.get_amplitudes()
+    with capi.Context.synthetic(O, V, seed, sc, precision=prec, switch_threshold=1e-6, final_fp64_sweeps=3) as ctx:
+        rmx = ctx.solve(1e-10, 200)
+        t1b, t2b = ctx.get_amplitudes()
+    assert rmx["lowp_iterations"] > 0 and rmx["fp64_iterations"] >= 3
+    assert abs(rmx["e_corr"] - r64["e_corr"]) <= 1e-7
+    assert np.max(np.abs(t2a - t2b)) <= 1e-6
