@@ -370,7 +370,7 @@ int ccsd_solve(ccsd_ctx* h, double tol, int max_iter, ccsd_result* out) {
       if (dE < std::max(c->opt.switch_threshold, tol)) {
         prec = CCSD_PREC_FP64;  // refine in FP64 from here on
         fp64_after_switch = 0;
-        diis_reset(c);
+        if (c->opt.diis_reset_on_switch) diis_reset(c);
       }
       continue;
     }

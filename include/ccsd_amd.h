@@ -63,6 +63,7 @@ typedef struct ccsd_options {
   int verbose;              /* print "Energi: E Steg: n Time/iter: s" lines like the reference */
   int device;               /* HIP device ordinal */
   size_t hbm_budget_bytes;  /* 0 = 90 % of the device's free memory */
+  int diis_reset_on_switch; /* mixed modes: clear the DIIS subspace at the switch to FP64 (default 0) */
 } ccsd_options;
 
 typedef struct ccsd_problem {
