@@ -1,13 +1,17 @@
 """Benchmark: CCSD amplitude iterations on one MI355X.
 
-A "step" is one CCSD amplitude update (right-hand side, Jacobi/DIIS update,
-tau and energy) on synthetic spin-orbital integrals resident in HBM.
+A "step" is one CCSD amplitude update on synthetic spin-orbital integrals
+already resident in HBM. It covers the right-hand side, the Jacobi/DIIS update,
+tau and the energy.
 
-Default workload: O=40 occupied / V=300 virtual spin orbitals (seed 40300),
-FP64.  Reported beside it: time-to-converge (|dE| <= 1e-7 Eh) of the FP64
-path and of the mixed-precision paths (FP32 and BF16 hi/lo x3 switching to
-FP64), the pp-ladder TF/s with its roofline fraction, and the CPU oracle timed
-on a bounded sample of the same workload.
+Workload: O=40 occupied / V=300 virtual spin orbitals, seed 40300, FP64.
+`value` is seconds per FP64 iteration. The same JSON line also reports:
+  * time-to-converge to |dE| <= 1e-7 Eh for FP64 and for the mixed paths
+    (FP32 and BF16 hi/lo x3, each finishing with 3 FP64 sweeps);
+  * per-iteration cost in each precision;
+  * pp-ladder TF/s and its roofline fraction (HIP events on the engine stream,
+    averaged over the timed steps);
+  * the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -21,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # dense MFMA TFLOP/s, MI355X
-HBM_PEAK_GBS = 8000.0
+METRIC = "CCSD s/iteration & time-to-converge (FP64 vs mixed) at |dE|<=1e-7 Eh; ladder TF/s"
 
 
 def p2(n):
@@ -42,15 +46,17 @@ def flops_per_iteration(O, V):
 
 
 def cpu_baseline(O, V, seed, scale, budget_s=15.0):
-    """CPU oracle on a bounded sample: the pp-ladder contraction of the FP64
-    oracle (numpy/BLAS) over a slice of virtual-pair columns, generated by the
-    same counter-based generator.  Extrapolated to one full iteration by flops."""
+    """CPU oracle on a bounded sample of the workload.
+
+    The sample is the pp-ladder contraction of the FP64 oracle (numpy BLAS,
+    all host threads) over a 64-column slice of <ef||ab>. The slice is made by
+    the same counter-based generator. The rate is extrapolated to one full
+    iteration by flop count."""
     from oracle import synth
     po, pv = p2(O), p2(V)
     rng = np.random.default_rng(0)
     tau = rng.standard_normal((po, pv)) * 1e-3
     ncols = 64
-    # columns [c0, c0+ncols) of <ef||ab>: all ef rows, a slice of ab
     t0 = time.perf_counter()
     blk = synth.packed_block("vvvv", O, V, seed, scale, rows=(0, ncols)).T  # symmetric block: rows == columns
     gen_s = time.perf_counter() - t0
@@ -61,7 +67,7 @@ def cpu_baseline(O, V, seed, scale, budget_s=15.0):
         reps += 1
         el = time.perf_counter() - t0
     per = el / reps
-    rate = 2.0 * po * pv * ncols / per  # flop/s
+    rate = 2.0 * po * pv * ncols / per
     total = sum(flops_per_iteration(O, V).values())
     threads = None
     try:
@@ -72,9 +78,36 @@ def cpu_baseline(O, V, seed, scale, budget_s=15.0):
     return {
         "value": total / rate, "unit": "s/iteration (extrapolated)", "cores": threads or os.cpu_count(),
         "kind": "port",
-        "sample": f"oracle pp-ladder tau[{po}x{pv}] @ <ef||ab>[{pv}x{ncols}] (numpy BLAS), {reps} reps; "
-                  f"rate {rate / 1e9:.1f} GFLOP/s scaled to {total:.3e} flop/iteration; block generation {gen_s:.1f}s",
+        "sample": f"oracle pp-ladder tau[{po}x{pv}] @ <ef||ab>[{pv}x{ncols}] (numpy BLAS), {reps} reps in "
+                  f"{el:.1f}s; {rate / 1e9:.1f} GFLOP/s scaled to {total:.3e} flop/iteration; "
+                  f"slice generation {gen_s:.1f}s",
     }
+
+
+def ladder_traffic(O, V, precision):
+    """HBM bytes per pp-ladder launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, calibrated on a copy kernel), if it covers
+    this workload; otherwise None."""
+    path = os.path.join(ROOT, "profiles", f"pp_ladder_o{O}_v{V}_{precision}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def timed_steps(ctx, steps, warmup, precision):
+    """Wall time per step and HIP-event kernel times averaged over the timed steps."""
+    ctx.iterate(warmup, precision)
+    acc = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.iterate(1, precision)
+        tm = ctx.timings()
+        acc = tm if acc is None else {k: acc[k] + tm[k] for k in tm}
+    el = time.perf_counter() - t0
+    return el / steps, {k: v / steps for k, v in acc.items()}
 
 
 def main():
@@ -85,15 +118,14 @@ def main():
     ap.add_argument("--O", type=int, default=40)
     ap.add_argument("--V", type=int, default=300)
     ap.add_argument("--seed", type=int, default=40300)
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lowp", action="store_true")
     args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only"}))
-        sys.exit(2)
-    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: one GPU only", file=sys.stderr)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU; the multi-GPU sharding is a design note (DESIGN.md section 7)",
+              file=sys.stderr)
         sys.exit(2)
 
     from ccsd_parallel_amd import capi
@@ -103,38 +135,52 @@ def main():
     total_flops = sum(fl.values())
 
     ctx = capi.Context.synthetic(O, V, seed, scale, precision="fp64", diis_dim=8)
-    ctx.iterate(args.warmup, args.precision)
-    t0 = time.perf_counter()
-    ctx.iterate(args.steps, args.precision)
-    el = time.perf_counter() - t0
-    tm = ctx.timings()
-    s_iter = el / max(1, args.steps)
+    s_iter, tm = timed_steps(ctx, args.steps, args.warmup, args.precision)
+    per_prec = {args.precision: {"s_per_iter": s_iter, "kernel_ms": tm}}
+    if not args.no_lowp:
+        for p in ("bf16x3", "fp32"):
+            if p == args.precision:
+                continue
+            ctx.reset()
+            s_p, tm_p = timed_steps(ctx, max(2, args.steps // 2), 1, p)
+            per_prec[p] = {"s_per_iter": s_p, "kernel_ms": tm_p}
+    ctx.close()
+    for p, v in per_prec.items():
+        ms = v["kernel_ms"]["pp_ladder"]
+        v["ladder_tflops"] = fl["pp_ladder"] / (ms * 1e-3) / 1e12 if ms > 0 else None
     prec = args.precision
-    ladder_tflops = fl["pp_ladder"] / (tm["pp_ladder"] * 1e-3) / 1e12 if tm["pp_ladder"] > 0 else None
+    ladder = per_prec[prec]["ladder_tflops"]
     out = {
-        "metric": "CCSD s/iteration (FP64 amplitude update) and time-to-converge, O=40 V=300 spin orbitals",
+        "metric": METRIC,
         "value": s_iter, "unit": "s/iteration", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": s_iter * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == "fp64" else prec, "data": "synthetic",
-        "config": {"workload": f"CCSD amplitude update, O={O} V={V} spin orbitals, synthetic seed {seed}, "
-                               f"sigma=1/sqrt(O*P2(V))", "O": O, "V": V},
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[prec], "data": "synthetic",
+        "config": {"workload": f"CCSD amplitude update, O={O} V={V} spin orbitals, synthetic <pq||rs> seed {seed}, "
+                               f"sigma=1/sqrt(O*P2(V))={scale:.3e}, DIIS(8)", "O": O, "V": V},
         "tflops_iteration": total_flops / s_iter / 1e12,
-        "kernel_ms": tm,
-        "ladder_tflops": ladder_tflops,
-        "roofline": {"bound": "mfma", "achieved": ladder_tflops, "peak": PEAK[prec], "unit": "TFLOP/s",
-                     "frac": (ladder_tflops / PEAK[prec]) if ladder_tflops else None, "traffic": None},
+        "flops_per_iteration": total_flops,
+        "per_precision": per_prec,
+        "ladder_tflops": ladder,
+        "roofline": {"bound": "mfma", "achieved": ladder, "peak": PEAK[prec], "unit": "TFLOP/s",
+                     "frac": (ladder / PEAK[prec]) if ladder else None,
+                     "traffic": ladder_traffic(O, V, prec), "traffic_unit": "bytes/launch",
+                     "kernel": "pp ladder GEMM tau[P2(O) x P2(V)] x <ef||ab>[P2(V) x P2(V)]",
+                     "algorithmic_flops": fl["pp_ladder"]},
     }
-    ctx.close()
 
     if not args.no_converge:
         conv = {}
         for p in ("fp64", "fp32", "bf16x3"):
-            c = capi.Context.synthetic(O, V, seed, scale, precision=p, diis_dim=8, switch_threshold=1e-5,
-                                       final_fp64_sweeps=3)
+            c = capi.Context.synthetic(O, V, seed, scale, precision=p, diis_dim=8, final_fp64_sweeps=3)
             r = c.solve(1e-7, 100, allow_unconverged=True)
             conv[p] = {k: r[k] for k in ("e_corr", "iterations", "converged", "seconds_total", "lowp_iterations",
                                          "fp64_iterations")}
             c.close()
+        c = capi.Context.synthetic(O, V, seed, scale, precision="fp64", diis_dim=8)
+        e_tight = c.solve(1e-11, 100, allow_unconverged=True)["e_corr"]
+        c.close()
+        for p in conv:
+            conv[p]["abs_dE_vs_fp64_1e-11"] = abs(conv[p]["e_corr"] - e_tight)
         for p in ("fp32", "bf16x3"):
             conv[p]["abs_dE_vs_fp64"] = abs(conv[p]["e_corr"] - conv["fp64"]["e_corr"])
             conv[p]["speedup_vs_fp64"] = conv["fp64"]["seconds_total"] / conv[p]["seconds_total"]

@@ -150,7 +150,8 @@ int plan_and_allocate(Ctx* c, int precision) {
   TRY(dalloc(c, &ws, (64l << 20) / 8));
   c->ws.ptr = ws;
   c->ws.bytes = 64ull << 20;
-  TRY(lowprec_allocate(c, precision));
+  // Low-precision operand copies are made on first use (lowprec_amplitudes),
+  // after the integrals are in HBM; plan_bytes above already reserved them.
   for (auto& e : c->ev) HIPTRY(hipEventCreate(&e));
   (void)N;
   return CCSD_OK;
@@ -264,15 +265,13 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
 
   // -- W_mbej stored transposed: WT[(j,b),(m,e)]
   {
-    // t_j^f <mb||ef>, batched over (m,b)
-    GemmDesc<double> g = G64(t1, V, 1, c->ovvv_u, 1, V, c->WT, V * ov, 1, O, V, V, 1.0, 0.0);
-    g.nb1 = (int)O; g.nb2 = (int)V;  // b1 = m, b2 = b
-    g.sB1 = V * VV; g.sB2 = VV;
-    g.sC1 = V; g.sC2 = ov;
-    TRY(run_gemm(c, g));
+    // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f as one tall GEMM (M = O V^2), staged in X
+    // (free until the ring GEMM); then WT = S^(permuted) + <mb||ej>.
+    double* S = c->X;
+    TRY(run_gemm(c, G64(c->ovvv_u, V, 1, t1, 1, V, S, O, 1, O * VV, O, V, 1.0, 0.0)));
+    k_fae_t1_from_s(c->Fae_t1, S, d, s);  // F_ae t1 part = -sum_m S[(m,a,e)][m]
+    k_wt_from_s(c->WT, S, c->G, d, s);
   }
-  k_fae_t1_extract(c->Fae_t1, c->WT, d, s);  // F_ae t1 part = -sum_m (t <mb||ef>)_{j=m, b=a}
-  k_axpy(c->WT, c->G, 1.0, ov * ov, s);      // + <mb||ej>
   {
     // + t_n^b W_mnje, batched over (j, m)
     GemmDesc<double> g = G64(t1, 1, V, c->W3u, V, 1, c->WT, ov, 1, V, V, O, 1.0, 1.0);

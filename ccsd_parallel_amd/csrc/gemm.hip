@@ -56,12 +56,20 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
   constexpr int TM = BM / WM / 16;
   constexpr int TN = BN / WN / 16;
   static_assert(TM >= 1 && TN >= 1, "wave tile too small");
-  // Row pitch padded by 16 elements keeps consecutive k-rows of a fetch in
-  // different LDS bank halves.
-  constexpr int LDA = BM + (sizeof(T) == 8 ? 16 : 32);
-  constexpr int LDB = BN + (sizeof(T) == 8 ? 16 : 32);
-  __shared__ T As[2][BK][LDA];
-  __shared__ T Bs[2][BK][LDB];
+  // LDS layouts.  A k-contiguous operand stays k-contiguous ([m][LDK]): the
+  // 16-byte stores of a row are contiguous and, with LDK = BK + 2 doubles, the
+  // 16 rows of one MFMA operand fetch fall in distinct banks.  An m/n-contiguous
+  // operand is stored [k][LDM]; the pitch BM + 16 puts consecutive k-rows of a
+  // fetch in opposite bank halves.
+  constexpr int LDK = BK + (sizeof(T) == 8 ? 2 : 4);
+  constexpr int LDMA = BM + (sizeof(T) == 8 ? 16 : 32);
+  constexpr int LDMB = BN + (sizeof(T) == 8 ? 16 : 32);
+  constexpr int ASZ = AK ? BM * LDK : BK * LDMA;
+  constexpr int BSZ = BNM ? BK * LDMB : BN * LDK;
+  __shared__ __attribute__((aligned(16))) T As[2][ASZ];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BSZ];
+  auto aidx = [](int m, int k) { return AK ? m * LDK + k : k * LDMA + m; };
+  auto bidx = [](int k, int n) { return BNM ? k * LDMB + n : n * LDK + k; };
 
   const GemmDesc<T>& d = p.d;
   // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8.
@@ -148,18 +156,18 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
       int u = it * NT + tid;
       if (u >= AU) break;
       if (AK) { int m = u / (BK / 2), kk = (u % (BK / 2)) * 2;
-                As[buf][kk][m] = ra[it][0]; As[buf][kk + 1][m] = ra[it][1]; }
+                As[buf][aidx(m, kk)] = ra[it][0]; As[buf][aidx(m, kk + 1)] = ra[it][1]; }
       else    { int kk = u / (BM / 2), m = (u % (BM / 2)) * 2;
-                As[buf][kk][m] = ra[it][0]; As[buf][kk][m + 1] = ra[it][1]; }
+                As[buf][aidx(m, kk)] = ra[it][0]; As[buf][aidx(m + 1, kk)] = ra[it][1]; }
     }
 #pragma unroll
     for (int it = 0; it < BIT; ++it) {
       int u = it * NT + tid;
       if (u >= BU) break;
       if (BNM) { int kk = u / (BN / 2), n = (u % (BN / 2)) * 2;
-                 Bs[buf][kk][n] = rb[it][0]; Bs[buf][kk][n + 1] = rb[it][1]; }
+                 Bs[buf][bidx(kk, n)] = rb[it][0]; Bs[buf][bidx(kk, n + 1)] = rb[it][1]; }
       else     { int n = u / (BK / 2), kk = (u % (BK / 2)) * 2;
-                 Bs[buf][kk][n] = rb[it][0]; Bs[buf][kk + 1][n] = rb[it][1]; }
+                 Bs[buf][bidx(kk, n)] = rb[it][0]; Bs[buf][bidx(kk + 1, n)] = rb[it][1]; }
     }
   };
 
@@ -185,9 +193,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
       const int kr = ks + (lane >> 4);
       T a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
+      for (int i = 0; i < TM; ++i) a[i] = As[cur][aidx(wm0 + i * 16 + (lane & 15), kr)];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[cur][kr][wn0 + j * 16 + (lane & 15)];
+      for (int j = 0; j < TN; ++j) b[j] = Bs[cur][bidx(kr, wn0 + j * 16 + (lane & 15))];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -230,6 +238,37 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
             *c = v;
           }
         }
+  }
+}
+
+// Many partials per output (long split-K on a small C): one workgroup per output
+// element, threads stride over the slices, fixed-order tree sum.
+template <typename T>
+__global__ void gemm_reduce_wide_kernel(GemmDesc<T> d, const T* __restrict__ W, int splits) {
+  const long MN = (long)d.M * d.N;
+  const int nbatch = d.nb1 * d.nb2;
+  const long e = blockIdx.x;  // element within one output matrix
+  const int bo = blockIdx.y;  // output batch (0 when reduce_batch)
+  const int nslices = d.reduce_batch ? splits * nbatch : splits;
+  T s = 0;
+  for (int q = threadIdx.x; q < nslices; q += blockDim.x) {
+    long slice = d.reduce_batch ? q : (long)q * nbatch + bo;
+    s += W[slice * MN + e];
+  }
+  __shared__ T sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int row = (int)(e / d.N), col = (int)(e % d.N);
+    long off = row * d.sCm + col * d.sCn;
+    if (!d.reduce_batch) off += (bo / d.nb2) * d.sC1 + (bo % d.nb2) * d.sC2;
+    T v = d.alpha * sh[0];
+    if (d.beta != T(0)) v += d.beta * d.C[off];
+    d.C[off] = v;
   }
 }
 
@@ -280,16 +319,20 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
   }
 }
 
-struct Cfg { int BM, BN, BK, WM, WN; float quality; };
-// quality: relative operand reuse of the tile shape (bigger tiles, fewer bytes per flop)
+// rate: FP64 TF/s of the config on a 12000^3 GEMM on MI355X (tile-sweep probe);
+// slots: resident workgroups per CU (LDS / register limited).
+struct Cfg { int BM, BN, BK, WM, WN; float rate; int slots; };
 constexpr Cfg kCfgs[] = {
-    {128, 128, 8, 2, 2, 1.00f},
-    {64, 128, 8, 2, 2, 0.90f},
-    {128, 64, 8, 2, 2, 0.90f},
-    {64, 64, 16, 2, 2, 0.75f},
-    {32, 64, 16, 1, 2, 0.60f},
-    {64, 32, 16, 2, 1, 0.60f},
+    {128, 256, 16, 2, 4, 54.4f, 1},
+    {256, 128, 16, 4, 2, 54.0f, 1},
+    {128, 128, 16, 2, 2, 39.1f, 2},
+    {64, 128, 16, 2, 2, 45.8f, 2},
+    {128, 64, 16, 2, 2, 45.6f, 2},
+    {64, 64, 16, 2, 2, 44.7f, 4},
+    {32, 64, 16, 1, 2, 25.0f, 4},
+    {64, 32, 16, 2, 1, 25.0f, 4},
 };
+constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 template <typename T, int BM, int BN, int BK, int WM, int WN>
@@ -305,11 +348,13 @@ hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStrea
 template <typename T>
 hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStream_t s) {
   switch (c) {
-    case 0: return launch_cfg<T, 128, 128, 8, 2, 2>(ka, AK, BNM, grid, s);
-    case 1: return launch_cfg<T, 64, 128, 8, 2, 2>(ka, AK, BNM, grid, s);
-    case 2: return launch_cfg<T, 128, 64, 8, 2, 2>(ka, AK, BNM, grid, s);
-    case 3: return launch_cfg<T, 64, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
-    case 4: return launch_cfg<T, 32, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
+    case 0: return launch_cfg<T, 128, 256, 16, 2, 4>(ka, AK, BNM, grid, s);
+    case 1: return launch_cfg<T, 256, 128, 16, 4, 2>(ka, AK, BNM, grid, s);
+    case 2: return launch_cfg<T, 128, 128, 16, 2, 2>(ka, AK, BNM, grid, s);
+    case 3: return launch_cfg<T, 64, 128, 16, 2, 2>(ka, AK, BNM, grid, s);
+    case 4: return launch_cfg<T, 128, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
+    case 5: return launch_cfg<T, 64, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
+    case 6: return launch_cfg<T, 32, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
     default: return launch_cfg<T, 64, 32, 16, 2, 1>(ka, AK, BNM, grid, s);
   }
 }
@@ -343,15 +388,22 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   const bool AK = !(d.sAm == 1 && d.sAk != 1);
   const bool BNM = !(d.sBk == 1 && d.sBn != 1);
 
+  // Pick the tile shape with the least modelled time: padding waste times
+  // wave quantisation over all CUs, divided by the shape's measured rate.
   int best = 0;
-  double best_score = -1;
+  double best_t = 1e300;
   for (int c = 0; c < kNumCfgs; ++c) {
     const Cfg& g = kCfgs[c];
     long tm = (d.M + g.BM - 1) / g.BM, tn = (d.N + g.BN - 1) / g.BN;
-    double eff = double(d.M) * d.N / (double(tm * g.BM) * double(tn * g.BN));
-    double fill = std::min(1.0, double(tm * tn * nbatch) / 512.0);
-    double score = eff * g.quality * (0.5 + 0.5 * fill);
-    if (score > best_score) { best_score = score; best = c; }
+    double pad = (double(tm * g.BM) * double(tn * g.BN)) / (double(d.M) * d.N);
+    double tiles = double(tm * tn) * nbatch;
+    double slots = double(kNumCus) * g.slots;
+    double rounds = std::ceil(tiles / slots);
+    double quant = rounds * slots / tiles;
+    // few tiles and a long K are fixed by split-K below; do not penalise them twice
+    if (tiles < slots && d.K >= 16 * g.BK * 8) quant = 1.0;
+    double t = pad * quant / g.rate;
+    if (t < best_t) { best_t = t; best = c; }
   }
   if (force_config >= 0 && force_config < kNumCfgs) best = force_config;
   const Cfg& g = kCfgs[best];
@@ -366,8 +418,9 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     splits = force_split;
   } else {
     long work = tiles * nbatch;
-    if (work < 1024) {
-      long want = (1024 + work - 1) / work;
+    const long target = 2L * kNumCus * g.slots;
+    if (work < target) {
+      long want = (target + work - 1) / work;
       long kmax = std::max(1, d.K / (g.BK * 8));
       splits = (int)std::max(1L, std::min(want, kmax));
     }
@@ -403,9 +456,17 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   hipError_t e = launch_any<T>(best, ka, AK, BNM, grid, stream);
   if (e != hipSuccess) return e;
   if (use_partial) {
-    long nout = (long)d.M * d.N * (d.reduce_batch ? 1 : nbatch);
-    int blocks = (int)std::min<long>((nout + 255) / 256, 4096);
-    hipLaunchKernelGGL(gemm_reduce_kernel<T>, dim3(blocks), dim3(256), 0, stream, d, (const T*)ws.ptr, splits);
+    const long MN = (long)d.M * d.N;
+    const int nout_b = d.reduce_batch ? 1 : nbatch;
+    const long slices = d.reduce_batch ? (long)splits * nbatch : splits;
+    if (slices >= 64 && MN <= (1L << 20) && nout_b <= 65535) {
+      hipLaunchKernelGGL(gemm_reduce_wide_kernel<T>, dim3((unsigned)MN, nout_b), dim3(256), 0, stream, d,
+                         (const T*)ws.ptr, splits);
+    } else {
+      long nout = MN * nout_b;
+      int blocks = (int)std::min<long>((nout + 255) / 256, 4096);
+      hipLaunchKernelGGL(gemm_reduce_kernel<T>, dim3(blocks), dim3(256), 0, stream, d, (const T*)ws.ptr, splits);
+    }
     e = hipGetLastError();
   }
   if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best};

@@ -226,6 +226,41 @@ __global__ void w2_finish_kernel(double* W2, const double* U, Dims d) {
   }
 }
 
+// WT[(j,b),(m,e)] = S[(m,b,e)][j] + G[(j,b),(m,e)], S = sum_f <mb||ef> t_j^f.
+// One workgroup per (m,b) and 64-wide chunk of e: the [e][j] block is staged in
+// LDS so that both the read of S and the write of WT are coalesced.
+__global__ void wt_from_s_kernel(double* WT, const double* S, const double* G, Dims d) {
+  extern __shared__ double tile[];  // [64][O+1]
+  const int O = d.O, V = d.V;
+  const long mb = blockIdx.x;
+  const int m = (int)(mb / V), b = (int)(mb % V);
+  const int e0 = blockIdx.y * 64;
+  const int ne = min(64, V - e0);
+  const double* src = S + ((long)mb * V + e0) * O;
+  for (int idx = threadIdx.x; idx < ne * O; idx += blockDim.x) {
+    const int el = idx / O, j = idx % O;
+    tile[el * (O + 1) + j] = src[idx];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < O * 64; idx += blockDim.x) {
+    const int j = idx / 64, el = idx % 64;
+    if (el >= ne) continue;
+    const long off = ((long)j * V + b) * d.ov + (long)m * V + e0 + el;
+    WT[off] = tile[el * (O + 1) + j] + G[off];
+  }
+}
+
+// Fae_t1[a][e] = -sum_m S[(m,a,e)][m]
+__global__ void fae_t1_from_s_kernel(double* F, const double* S, Dims d) {
+  const long n = (long)d.V * d.V;
+  GRID_LOOP(idx, n) {
+    long a = idx / d.V, e = idx % d.V;
+    double s = 0.0;
+    for (int m = 0; m < d.O; ++m) s += S[(((long)m * d.V + a) * d.V + e) * d.O + m];
+    F[idx] = -s;
+  }
+}
+
 // Fae_t1[a][e] = -sum_m WT[(m,a),(m,e)]  (WT holding sum_f t_j^f <mb||ef> only)
 __global__ void fae_t1_kernel(double* F, const double* WT, Dims d) {
   const long n = (long)d.V * d.V;
@@ -237,25 +272,22 @@ __global__ void fae_t1_kernel(double* F, const double* WT, Dims d) {
   }
 }
 
-// Fmi[m][i] = (1-delta) f_mi + sum_ne t_n^e <mn||ie>
+__device__ __forceinline__ double block_sum(double v);
+
+// Fmi[m][i] = (1-delta) f_mi + sum_ne t_n^e <mn||ie>; one workgroup per (m, i),
+// threads over (n, e) with a fixed-order block reduction.
 __global__ void fmi_init_kernel(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d) {
   const long N = d.O + d.V;
-  const long n = (long)d.O * d.O;
-  GRID_LOOP(idx, n) {
-    int m = (int)(idx / d.O), i = (int)(idx % d.O);
-    double s = (m == i) ? 0.0 : f[(long)m * N + i];
-    for (int nn = 0; nn < d.O; ++nn) {
-      long k;
-      double sg = sgn_pair(m, nn, k);
-      if (sg == 0.0) continue;
-      const double* row = ooov + k * d.ov + (long)i * d.V;
-      const double* tn = t1 + (long)nn * d.V;
-      double acc = 0.0;
-      for (int e = 0; e < d.V; ++e) acc += tn[e] * row[e];
-      s += sg * acc;
-    }
-    Fmi[idx] = s;
+  const int m = blockIdx.x / d.O, i = blockIdx.x % d.O;
+  double acc = 0.0;
+  for (long ne = threadIdx.x; ne < d.ov; ne += blockDim.x) {
+    const int nn = (int)(ne / d.V), e = (int)(ne % d.V);
+    long k;
+    const double sg = sgn_pair(m, nn, k);
+    if (sg != 0.0) acc += sg * t1[ne] * ooov[k * d.ov + (long)i * d.V + e];
   }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) Fmi[blockIdx.x] = acc + ((m == i) ? 0.0 : f[(long)m * N + i]);
 }
 
 // Fmi[m][i] += sum_e Fme[m][e] t1[i][e]
@@ -524,11 +556,18 @@ void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s) {
   LAUNCH(w3_unpack_kernel, (long)d.O * d.O * d.O * d.V, W3u, W3, d);
 }
 void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s) { LAUNCH(w2_finish_kernel, d.po * d.ov, W2, U, d); }
+void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s) {
+  dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
+  hipLaunchKernelGGL(wt_from_s_kernel, grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s, WT, S, G, d);
+}
+void k_fae_t1_from_s(double* F, const double* S, Dims d, hipStream_t s) {
+  LAUNCH(fae_t1_from_s_kernel, (long)d.V * d.V, F, S, d);
+}
 void k_fae_t1_extract(double* F, const double* WT, Dims d, hipStream_t s) {
   LAUNCH(fae_t1_kernel, (long)d.V * d.V, F, WT, d);
 }
 void k_fmi_init(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d, hipStream_t s) {
-  LAUNCH(fmi_init_kernel, (long)d.O * d.O, Fmi, f, ooov, t1, d);
+  hipLaunchKernelGGL(fmi_init_kernel, dim3(d.O * d.O), dim3(kThreads), 0, s, Fmi, f, ooov, t1, d);
 }
 void k_fmi_finish(double* Fmi, const double* Fme, const double* t1, Dims d, hipStream_t s) {
   LAUNCH(fmi_finish_kernel, (long)d.O * d.O, Fmi, Fme, t1, d);

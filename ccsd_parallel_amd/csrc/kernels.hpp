@@ -54,6 +54,9 @@ void k_w3_init(double* W3, const double* ooov, Dims d, hipStream_t s);
 void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s);
 void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s);
 void k_fae_t1_extract(double* Fae_t1, const double* WT, Dims d, hipStream_t s);
+// S[(m,b,e)][j] = sum_f <mb||ef> t_j^f  ->  WT[(j,b),(m,e)] = S + <mb||ej>;  F_ae t1 part from S
+void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s);
+void k_fae_t1_from_s(double* Fae_t1, const double* S, Dims d, hipStream_t s);
 void k_fmi_init(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d, hipStream_t s);
 void k_fmi_finish(double* Fmi, const double* Fme, const double* t1, Dims d, hipStream_t s);
 void k_fae_init(double* Fae, const double* f, const double* Fae_t1, const double* Fme, const double* t1, Dims d,

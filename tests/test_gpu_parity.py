@@ -158,6 +158,23 @@ def test_two_electron_fci():
     assert abs(r["e_corr"] - e_fci) <= 1e-10
 
 
+@pytest.mark.parametrize("prec,floor", [("fp32", 1e-7), ("bf16x3", 2e-6)])
+def test_low_precision_fixed_point_is_near_fp64(prec, floor):
+    """Never switching to FP64, the low-precision iteration must still converge
+    to within its rounding floor of the FP64 energy (catches a wrong or stale
+    low-precision operand, which the FP64 refinement would otherwise hide)."""
+    capi = _ctx()
+    O, V, seed = 10, 40, 4321
+    sc = norm_scale(O, V)
+    with capi.Context.synthetic(O, V, seed, sc) as ctx:
+        e64 = ctx.solve(1e-11, 200)["e_corr"]
+    with capi.Context.synthetic(O, V, seed, sc, precision=prec, switch_threshold=1e-30) as ctx:
+        r = ctx.solve(1e-30, 40, allow_unconverged=True)
+        tr = ctx.energy_trace()
+    assert r["fp64_iterations"] == 0
+    assert abs(tr[-1] - e64) <= floor * max(1.0, abs(e64))
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mixed_precision_converges_to_fp64(prec):
     capi = _ctx()
