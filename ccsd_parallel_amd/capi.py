@@ -23,7 +23,7 @@ EXPORTED = (
     "ccsd_default_options", "ccsd_create", "ccsd_create_synthetic", "ccsd_solve", "ccsd_iterate", "ccsd_reset",
     "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
-    "ccsd_selftest_gemm",
+    "ccsd_selftest_gemm", "ccsd_update_integrals",
 )
 
 
@@ -95,6 +95,7 @@ def load():
     lib.ccsd_strerror.restype = C.c_char_p
     lib.ccsd_debug_get.argtypes = [vp, C.c_char_p, dp, C.c_long]
     lib.ccsd_debug_get.restype = C.c_long
+    lib.ccsd_update_integrals.argtypes = [vp, C.c_int, C.c_long, C.POINTER(C.c_long), dp]
     lib.ccsd_selftest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_double,
                                        dp, C.c_int]
     _lib = lib
@@ -234,15 +235,27 @@ class Context:
             raise CCSDError(int(rc), "ccsd_debug_get")
         return out
 
-    def backward(self):
+    BLOCK_IDS = {"oooo": 0, "ooov": 1, "oovv": 2, "ovov": 3, "ovvv": 4, "vvvv": 5, "f": 6}
+
+    def update_integrals(self, block, index, delta):
+        """Add delta to entries (flat indices) of one input array on the device."""
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        dl = np.ascontiguousarray(delta, dtype=np.float64)
+        _check(self.lib.ccsd_update_integrals(self.h, self.BLOCK_IDS[block], len(idx),
+                                              idx.ctypes.data_as(C.POINTER(C.c_long)), _ptr(dl)),
+               "ccsd_update_integrals")
+
+    def backward(self, allow_unconverged=False):
         O, V = self.O, self.V
         N = O + V
         g = dict(f=np.zeros((N, N)), oooo=np.zeros((self.po, self.po)), ooov=np.zeros((self.po, self.ov)),
                  oovv=np.zeros((self.po, self.pv)), ovov=np.zeros((self.ov, self.ov)),
                  ovvv=np.zeros((self.ov, self.pv)), vvvv=np.zeros((self.pv, self.pv)))
         info = GradInfo()
-        _check(self.lib.ccsd_backward(self.h, *[_ptr(g[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv",
-                                                                      "vvvv")], C.byref(info)), "ccsd_backward")
+        rc = self.lib.ccsd_backward(self.h, *[_ptr(g[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv",
+                                                                    "vvvv")], C.byref(info))
+        if rc != 0 and not (allow_unconverged and rc == -5):
+            raise CCSDError(rc, "ccsd_backward")
         return g, {k: getattr(info, k) for k, _ in info._fields_}
 
 

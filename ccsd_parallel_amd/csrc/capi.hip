@@ -120,8 +120,9 @@ void diis_reset(Ctx* c) {
   std::fill(c->B.begin(), c->B.end(), 0.0);
 }
 
-// Bordered Pulay system with diagonal scaling; false if singular or wild.
-bool diis_coefficients(const std::vector<double>& Bfull, int m, std::vector<double>& coef) {
+}  // namespace
+
+bool diis_solve_bordered(const std::vector<double>& Bfull, int m, std::vector<double>& coef) {
   const int n = m + 1;
   std::vector<double> A((size_t)n * n, -1.0), b(n, 0.0), s(n, 1.0);
   double dmax = 0.0;
@@ -167,6 +168,8 @@ bool diis_coefficients(const std::vector<double>& Bfull, int m, std::vector<doub
   return true;
 }
 
+namespace {
+
 // One amplitude update in the given precision.  On return c->amp holds the new
 // (possibly extrapolated) amplitudes; *dt2 = max |t2_new - t2_old|.
 int update_once(Ctx* c, int prec, bool allow_diis, bool* diis_fell_back, double* dt2) {
@@ -209,7 +212,7 @@ int update_once(Ctx* c, int prec, bool allow_diis, bool* diis_fell_back, double*
       std::vector<double> Bc((size_t)cnt * cnt), coef;
       for (int i = 0; i < cnt; ++i)
         for (int j = 0; j < cnt; ++j) Bc[(size_t)i * cnt + j] = c->B[(size_t)i * m + j];
-      if (diis_coefficients(Bc, cnt, coef)) {
+      if (diis_solve_bordered(Bc, cnt, coef)) {
         HIPTRY(hipMemcpyAsync(c->dcoef, coef.data(), cnt * 8, hipMemcpyHostToDevice, s));
         k_lincomb(c->amp, c->dvec, c->nvec, c->dcoef, cnt, c->nvec, s);
         HIPTRY(hipGetLastError());
@@ -515,6 +518,58 @@ long ccsd_debug_get(ccsd_ctx* h, const char* name, double* out, long cap) {
     return it.cnt;
   }
   return CCSD_ERR_ARG;
+}
+
+int ccsd_update_integrals(ccsd_ctx* h, int block, long n, const long* index, const double* delta) {
+  if (!h || n < 0 || (n > 0 && (!index || !delta)) || block < 0 || block > 6) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  const Dims& d = c->d;
+  const long N = d.O + d.V;
+  const long sizes[7] = {d.po * d.po, d.po * d.ov, d.po * d.pv, d.ov * d.ov, d.ov * d.pv, d.pv * d.pv, N * N};
+  for (long k = 0; k < n; ++k) {
+    if (index[k] < 0 || index[k] >= sizes[block]) return CCSD_ERR_ARG;
+    if (!std::isfinite(delta[k])) return CCSD_ERR_NONFINITE;
+  }
+  if (n == 0) return CCSD_OK;
+  HIPTRY(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (block == 6) {
+    std::vector<double> f = c->h_f;
+    for (long k = 0; k < n; ++k) f[index[k]] += delta[k];
+    std::swap(f, c->h_f);
+    int rc = check_fock(c);
+    if (rc != CCSD_OK) { std::swap(f, c->h_f); return rc; }
+    std::vector<double> fov((size_t)d.ov);
+    for (long i = 0; i < d.O; ++i)
+      for (long a = 0; a < d.V; ++a) fov[i * d.V + a] = c->h_f[i * N + d.O + a];
+    HIPTRY(hipMemcpyAsync(c->f, c->h_f.data(), c->h_f.size() * 8, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(c->fov, fov.data(), fov.size() * 8, hipMemcpyHostToDevice, s));
+  } else {
+    double* targets[6] = {c->oooo, c->ooov, c->oovv, c->G, c->ovvv, c->vvvv};
+    long* didx = nullptr;
+    double* dval = nullptr;
+    HIPTRY(hipMalloc(&didx, n * sizeof(long)));
+    if (hipMalloc(&dval, n * 8) != hipSuccess) { (void)hipFree(didx); return CCSD_ERR_HIP; }
+    hipError_t e = hipMemcpyAsync(didx, index, n * sizeof(long), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dval, delta, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+      k_scatter_add(targets[block], didx, dval, n, block == 3 ? 1 : 0, d, s);
+      e = hipStreamSynchronize(s);
+    }
+    (void)hipFree(didx);
+    (void)hipFree(dval);
+    if (e != hipSuccess) { c->last_error = hipGetErrorString(e); return CCSD_ERR_HIP; }
+    if (block == 2) {  // derived <mn||ef> layouts
+      k_build_Doovv(c->Doovv, c->oovv, d, s);
+      k_build_oovv_ou(c->oovv_ou, c->oovv, d, s);
+      k_build_oovv_vu(c->oovv_vu, c->oovv, d, s);
+    }
+    if (block == 4) k_build_ovvv_u(c->ovvv_u, c->ovvv, d, s);
+  }
+  lowprec_free(c);  // low-precision operand copies are stale; remade on next use
+  c->solved = false;
+  HIPTRY(hipStreamSynchronize(s));
+  return CCSD_OK;
 }
 
 int ccsd_backward(ccsd_ctx* h, double* dE_dfock, double* dE_doooo, double* dE_dooov, double* dE_doovv,

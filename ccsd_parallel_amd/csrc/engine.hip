@@ -19,16 +19,6 @@ namespace ccsd {
 
 namespace {
 
-GemmDesc<double> G64(const double* A, long sAm, long sAk, const double* B, long sBk, long sBn, double* C, long sCm,
-                     long sCn, long M, long N, long K, double alpha, double beta) {
-  GemmDesc<double> g;
-  g.A = A; g.B = B; g.C = C;
-  g.M = (int)M; g.N = (int)N; g.K = (int)K;
-  g.sAm = sAm; g.sAk = sAk; g.sBk = sBk; g.sBn = sBn; g.sCm = sCm; g.sCn = sCn;
-  g.alpha = alpha; g.beta = beta;
-  return g;
-}
-
 #define TRY(x)                         \
   do {                                 \
     int _rc = (x);                     \

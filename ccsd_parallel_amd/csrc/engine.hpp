@@ -10,7 +10,8 @@
 
 namespace ccsd {
 
-struct LowPrec;  // FP32 / BF16 operand copies for the mixed-precision GEMMs
+struct LowPrec;   // FP32 / BF16 operand copies for the mixed-precision GEMMs
+struct AdjState;  // adjoint (Lambda) buffers, allocated by ccsd_backward
 
 struct Ctx {
   Dims d{};
@@ -47,6 +48,7 @@ struct Ctx {
   GemmWorkspace ws;
 
   LowPrec* lp = nullptr;
+  AdjState* adj = nullptr;
 
   // state
   bool solved = false;
@@ -63,11 +65,25 @@ int plan_and_allocate(Ctx* c, int precision);
 int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool time_it);
 int device_energy(Ctx* c, const double* t1, const double* t2, double* e);
 size_t plan_bytes(int O, int V, int precision, int diis_m);
+// Bordered Pulay system (B_ij = <e_i|e_j>, -1 border, rhs -1) with diagonal
+// scaling; false when singular or the coefficients are not finite / huge.
+bool diis_solve_bordered(const std::vector<double>& B, int m, std::vector<double>& coef);
+
+// Row-major strided GEMM descriptor C(MxN) = alpha A(MxK) B(KxN) + beta C.
+inline GemmDesc<double> G64(const double* A, long sAm, long sAk, const double* B, long sBk, long sBn, double* C,
+                            long sCm, long sCn, long M, long N, long K, double alpha, double beta) {
+  GemmDesc<double> g;
+  g.A = A; g.B = B; g.C = C;
+  g.M = (int)M; g.N = (int)N; g.K = (int)K;
+  g.sAm = sAm; g.sAk = sAk; g.sBk = sBk; g.sBn = sBn; g.sCm = sCm; g.sCn = sCn;
+  g.alpha = alpha; g.beta = beta;
+  return g;
+}
 
 // GEMM helper with error propagation
 template <typename T>
-inline int run_gemm(Ctx* c, const GemmDesc<T>& g) {
-  hipError_t e = gemm<T>(g, c->ws, c->stream);
+inline int run_gemm(Ctx* c, const GemmDesc<T>& g, const GemmWorkspace* ws = nullptr) {
+  hipError_t e = gemm<T>(g, ws ? *ws : c->ws, c->stream);
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;

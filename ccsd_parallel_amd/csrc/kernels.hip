@@ -462,6 +462,21 @@ __global__ void sum_partials_kernel(double* out, const double* partial, int npar
   if (threadIdx.x == 0) out[s] = sh[0];
 }
 
+// Sequential per index would race on duplicates; one thread walks the list in order.
+__global__ void scatter_add_kernel(double* x, const long* idx, const double* delta, long n, int to_g, Dims d) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (long k = 0; k < n; ++k) {
+    long t = idx[k];
+    if (to_g) {
+      const long r = t / d.ov, c = t % d.ov;
+      const long m = r / d.V, b = r % d.V, j = c / d.V, e = c % d.V;
+      x[(m * d.V + e) * d.ov + j * d.V + b] -= delta[k];
+    } else {
+      x[t] += delta[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- synthetic generator
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -609,6 +624,10 @@ void k_lincomb(double* y, const double* X, long ldx, const double* c, int nv, lo
 }
 void k_sum_partials(double* out, const double* partial, int nparts, int nsums, hipStream_t s) {
   hipLaunchKernelGGL(sum_partials_kernel, dim3(nsums), dim3(kThreads), 0, s, out, partial, nparts, nsums);
+}
+
+void k_scatter_add(double* x, const long* idx, const double* delta, long n, int to_g, Dims d, hipStream_t s) {
+  hipLaunchKernelGGL(scatter_add_kernel, dim3(1), dim3(64), 0, s, x, idx, delta, n, to_g, d);
 }
 
 void k_synth_block(double* out, int block, long r0, long r1, Dims d, uint64_t seed, double scale, hipStream_t s) {

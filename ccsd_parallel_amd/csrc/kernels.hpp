@@ -85,6 +85,10 @@ void k_multidot_partials(double* partial, const double* X, long ldx, int nv, con
 void k_lincomb(double* y, const double* X, long ldx, const double* c, int nv, long n, hipStream_t s);
 void k_sum_partials(double* out, const double* partial, int nparts, int nsums, hipStream_t s);
 
+// x[idx[k]] += delta[k]  (idx, delta on device); for ovov deltas applied to G:
+// G[(m,e),(j,b)] -= delta at ovov index (m,b),(j,e) when to_g is set.
+void k_scatter_add(double* x, const long* idx, const double* delta, long n, int to_g, Dims d, hipStream_t s);
+
 // -- synthetic generator (counter-based; restates oracle/synth.py) ------------------------
 // block id: 0 oooo, 1 ooov, 2 oovv, 3 ovov, 4 ovvv, 5 vvvv.  rows [r0, r1) of the block.
 void k_synth_block(double* out, int block, long r0, long r1, Dims d, uint64_t seed, double scale, hipStream_t s);

@@ -143,6 +143,13 @@ int ccsd_energy_trace(ccsd_ctx* c, double* out, int cap);
 int ccsd_backward(ccsd_ctx* c, double* dE_dfock, double* dE_doooo, double* dE_dooov, double* dE_doovv,
                   double* dE_dovov, double* dE_dovvv, double* dE_dvvvv, ccsd_grad_info* info);
 
+/* Add delta[k] to n entries of one input array in place (block 0 oooo,
+ * 1 ooov, 2 oovv, 3 ovov, 4 ovvv, 5 vvvv: flat index into the packed block;
+ * 6: flat index into the N x N Fock matrix).  Derived layouts are rebuilt and
+ * the amplitudes are kept (warm start for the next solve).  The caller keeps
+ * symmetric blocks symmetric. */
+int ccsd_update_integrals(ccsd_ctx* c, int block, long n, const long* index, const double* delta);
+
 /* Device bytes the context holds (or would hold for O, V before create). */
 size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision);
 

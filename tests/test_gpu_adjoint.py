@@ -1,0 +1,131 @@
+"""GPU adjoint (ccsd_backward) against the CPU adjoint oracle and finite differences.
+
+Gradients are with respect to the packed input arrays (the Fock matrix and the
+six <pq||rs> blocks), symmetrised over duplicate entries, so a symmetric
+perturbation Delta gives dE/dh = sum(g * Delta) over the stored entries.
+"""
+import numpy as np
+import pytest
+
+from oracle import adjoint_ref, synth
+
+pytestmark = pytest.mark.gpu
+KEYS = ("f",) + synth.BLOCKS
+
+
+def _capi():
+    from ccsd_parallel_amd import capi
+    return capi
+
+
+def _prob(O, V, seed, noncanon=False):
+    prob = synth.make_problem(O, V, seed, scale=1.0 / np.sqrt(O * synth.p2(V)))
+    if noncanon:
+        rng = np.random.default_rng(seed)
+        a = rng.standard_normal(prob["f"].shape)
+        prob["f"] = prob["f"] + 0.01 * (a + a.T)
+    return prob
+
+
+@pytest.mark.parametrize("O,V,seed,noncanon", [(4, 8, 31, False), (6, 12, 32, True), (8, 16, 33, False)])
+def test_backward_matches_cpu_adjoint(O, V, seed, noncanon):
+    capi = _capi()
+    prob = _prob(O, V, seed, noncanon)
+    E, gref, _, linfo = adjoint_ref.adjoint(prob, toler=1e-13)
+    with capi.Context.from_problem(prob, diis_dim=8) as ctx:
+        r = ctx.solve(1e-13, 300)
+        g, info = ctx.backward()
+    assert abs(r["e_corr"] - E) < 1e-11
+    assert info["converged"] == 1
+    for k in KEYS:
+        scale = max(1e-3, np.abs(gref[k]).max())
+        err = np.abs(g[k] - gref[k]).max()
+        assert err <= 1e-8 * scale, (k, err, scale)  # Lambda converged to 1e-10
+
+
+def test_backward_dense_finite_differences():
+    """16 dense random symmetric directions on the host-built problem, h = 1e-5."""
+    capi = _capi()
+    O, V, seed = 6, 16, 34
+    prob = _prob(O, V, seed)
+    with capi.Context.from_problem(prob, diis_dim=8) as ctx:
+        ctx.solve(1e-13, 300)
+        g, _ = ctx.backward()
+    h = 1e-5
+    worst = 0.0
+    for k in range(16):
+        d = synth.make_problem(O, V, 5000 + k, scale=1.0)
+        rng = np.random.default_rng(k)
+        a = rng.standard_normal(d["f"].shape)
+        d["f"] = 0.5 * (a + a.T)
+        pred = sum(float(np.sum(g[b] * d[b])) for b in KEYS)
+        es = []
+        for sgn in (1, -1):
+            p = dict(prob, **{b: prob[b] + sgn * h * d[b] for b in KEYS})
+            with capi.Context.from_problem(p, diis_dim=8) as c2:
+                es.append(c2.solve(1e-14, 300)["e_corr"])
+        fd = (es[0] - es[1]) / (2 * h)
+        worst = max(worst, abs(fd - pred) / max(1e-12, abs(fd)))
+    assert worst <= 1e-6, worst
+
+
+def _sparse_direction(rng, O, V, kind, n):
+    """Flat indices and deltas of one symmetric-consistent sparse direction."""
+    N = O + V
+    po, pv, ov = synth.p2(O), synth.p2(V), O * V
+    shapes = dict(f=(N, N), oooo=(po, po), ooov=(po, ov), oovv=(po, pv), ovov=(ov, ov), ovvv=(ov, pv),
+                  vvvv=(pv, pv))
+    r, c = shapes[kind]
+    if kind == "f":
+        i = rng.integers(0, N, size=N * N // 4)
+        j = rng.integers(0, N, size=N * N // 4)
+    else:
+        i = rng.integers(0, r, size=n)
+        j = rng.integers(0, c, size=n)
+    v = rng.standard_normal(len(i)) * (1.0 if kind == "f" else 10.0)
+    if kind in ("f", "oooo", "ovov", "vvvv"):  # symmetric blocks: add the mirrored entry too
+        idx = np.concatenate([i * c + j, j * c + i])
+        val = np.concatenate([v, v])
+    else:
+        idx, val = i * c + j, v
+    return idx.astype(np.int64), val
+
+
+@pytest.mark.timeout(600)
+def test_backward_finite_differences_o20_v180():
+    """64 seeded directions at O=20 / V=180 (central differences, h = 1e-5),
+    applied on the device with ccsd_update_integrals; each direction perturbs one
+    input array (cycling over f and the six blocks)."""
+    capi = _capi()
+    O, V, seed = 20, 180, 20180
+    sc = 1.0 / np.sqrt(O * synth.p2(V))
+    h = 1e-5
+    kinds = list(KEYS)
+    with capi.Context.synthetic(O, V, seed, sc, diis_dim=8) as ctx:
+        ctx.solve(1e-13, 200)
+        t1, t2 = ctx.get_amplitudes()
+        g, info = ctx.backward()
+        assert info["converged"] == 1
+        rng = np.random.default_rng(64)
+        rel, rows = [], []
+        for k in range(64):
+            kind = kinds[k % len(kinds)]
+            idx, val = _sparse_direction(rng, O, V, kind, 20000)
+            pred = float(np.sum(g[kind].ravel()[idx] * val))
+            es = []
+            for step in (h, -2 * h):
+                ctx.update_integrals(kind, idx, step * val)
+                ctx.set_amplitudes(t1, t2)
+                es.append(ctx.solve(1e-15, 300, allow_unconverged=True)["e_corr"])
+            ctx.update_integrals(kind, idx, h * val)  # restore
+            fd = (es[0] - es[1]) / (2 * h)
+            rel.append(abs(fd - pred) / max(abs(fd), 1e-30))
+            rows.append((kind, fd, pred))
+        rel = np.array(rel)
+    for kind, fd, pred in sorted(rows, key=lambda x: -abs(x[1] - x[2]) / max(abs(x[1]), 1e-30))[:5]:
+        print("worst: %s fd %.6e pred %.6e |d| %.2e" % (kind, fd, pred, abs(fd - pred)))
+    print("FD relative errors: max %.2e median %.2e" % (rel.max(), np.median(rel)))
+    # energies converge to ~1e-14 Eh, so a central difference carries ~1e-9 absolute noise
+    for kind, fd, pred in rows:
+        assert abs(fd - pred) <= max(1e-6 * abs(fd), 1e-9), (kind, fd, pred)
+    assert np.median(rel) <= 1e-6
