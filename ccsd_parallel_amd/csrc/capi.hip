@@ -614,7 +614,7 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
     rc = CCSD_ERR_HIP;
   }
   if (rc == CCSD_OK) {
-    if (precision == CCSD_PREC_FP64) {
+    if ((precision & 0xff) == CCSD_PREC_FP64) {
       (void)hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice);
       (void)hipMemcpy(dB, B, (size_t)K * N * 8, hipMemcpyHostToDevice);
       (void)hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice);
@@ -624,7 +624,8 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
       g.sBk = tb ? 1 : N; g.sBn = tb ? K : 1;
       g.sCm = N; g.sCn = 1;
       g.alpha = alpha; g.beta = beta;
-      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s) != hipSuccess) rc = CCSD_ERR_HIP;
+      const int cfg = ((precision >> 8) & 0xff) - 1;  // forced tile configuration, -1 = automatic
+      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, -1) != hipSuccess) rc = CCSD_ERR_HIP;
     } else {
       // low-precision path wants A as [M][K] and B as [N][K]
       std::vector<double> Ak((size_t)M * K), Bk((size_t)N * K);

@@ -46,7 +46,9 @@ template <typename T>
 struct KArgs {
   GemmDesc<T> d;
   T* partial;      // [splits][nbatch][M][N] when non-null
-  int tiles_m, tiles_n, tiles_pad;  // tiles_pad: tiles rounded up to a multiple of 8
+  // tiles_pad: tiles rounded up to a multiple of 8 when the XCD-aware order is
+  // used; equal to tiles (not a multiple of 8) for the plain order
+  int tiles_m, tiles_n, tiles_pad;
   int splits, kchunk;
 };
 
@@ -72,11 +74,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
   auto bidx = [](int k, int n) { return BNM ? k * LDMB + n : n * LDK + k; };
 
   const GemmDesc<T>& d = p.d;
-  // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8.
+  // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8, so
+  // XCD j takes the contiguous tile range [j*gx/8, (j+1)*gx/8).  With few tiles
+  // (gx not a multiple of 8) the plain order spreads the batches and K-splits
+  // over all XCDs instead of stacking every real tile on the low ones.
   const int gx = p.tiles_pad;
-  const int xcd = blockIdx.x & 7;
-  const int tid_in_xcd = blockIdx.x >> 3;
-  const int tile = xcd * (gx >> 3) + tid_in_xcd;
+  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
   if (tile >= p.tiles_m * p.tiles_n) return;
   const int tm = tile % p.tiles_m;
   const int tn = tile / p.tiles_m;
@@ -331,6 +334,11 @@ constexpr Cfg kCfgs[] = {
     {64, 64, 16, 2, 2, 44.7f, 4},
     {32, 64, 16, 1, 2, 25.0f, 4},
     {64, 32, 16, 2, 1, 25.0f, 4},
+    // 48-wide tiles for the O = 40 edges (M or N of 40 fills 83% instead of 62%)
+    {128, 48, 16, 4, 1, 42.0f, 2},
+    {48, 128, 16, 1, 4, 42.0f, 2},
+    {48, 64, 16, 1, 2, 38.0f, 4},
+    {64, 48, 16, 2, 1, 38.0f, 4},
 };
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
@@ -355,7 +363,11 @@ hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, h
     case 4: return launch_cfg<T, 128, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
     case 5: return launch_cfg<T, 64, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
     case 6: return launch_cfg<T, 32, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
-    default: return launch_cfg<T, 64, 32, 16, 2, 1>(ka, AK, BNM, grid, s);
+    case 7: return launch_cfg<T, 64, 32, 16, 2, 1>(ka, AK, BNM, grid, s);
+    case 8: return launch_cfg<T, 128, 48, 16, 4, 1>(ka, AK, BNM, grid, s);
+    case 9: return launch_cfg<T, 48, 128, 16, 1, 4>(ka, AK, BNM, grid, s);
+    case 10: return launch_cfg<T, 48, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
+    default: return launch_cfg<T, 64, 48, 16, 2, 1>(ka, AK, BNM, grid, s);
   }
 }
 
@@ -448,7 +460,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   ka.partial = use_partial ? (T*)ws.ptr : nullptr;
   ka.tiles_m = tiles_m;
   ka.tiles_n = tiles_n;
-  ka.tiles_pad = (int)(((tiles + 7) / 8) * 8);
+  ka.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
   ka.splits = splits;
   ka.kchunk = kchunk;
   if (ka.tiles_pad <= 0 || splits > 65535) return hipErrorInvalidValue;

@@ -106,8 +106,8 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * LDK];
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * LDK];
 
-  const int xcd = blockIdx.x & 7;
-  const int tile = xcd * (p.tiles_pad >> 3) + (blockIdx.x >> 3);
+  const int gx = p.tiles_pad;  // XCD-aware order when a multiple of 8 (see bf16x3_nt)
+  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
   if (tile >= p.tiles_m * p.tiles_n) return;
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
@@ -194,17 +194,171 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
       }
 }
 
+// Large-tile variant: 8 waves, each a 64 x 128 sub-tile (4 x 8 MFMA tiles, 128
+// accumulator registers), so one 256 x 256 (or 128 x 512) block tile reaches
+// ~190 flop per byte loaded -- the 128 x 128 tile above needs ~25 TB/s of L2
+// bandwidth to feed 2.5 PF/s.  Per k-step the A fragments of the wave are kept
+// in registers and the B fragments are streamed one 16-column tile at a time.
+// DBUF: double-buffered LDS (one barrier per k-tile; 160 KB at 256 x 256).
+template <int BM, int BN, int WM, int WN, bool DBUF>
+__global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_big_kernel(Bf16Args p) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BK = 32;
+  constexpr int LDK = BK + 8;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int CA = 2 * BM * 4 / NT;
+  constexpr int CB = 2 * BN * 4 / NT;
+  static_assert(CA * NT == 2 * BM * 4 && CB * NT == 2 * BN * 4, "tile/threads mismatch");
+  constexpr int NBUF = DBUF ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[NBUF][2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[NBUF][2][BN * LDK];
+
+  const int gx = p.tiles_pad;  // XCD-aware order when a multiple of 8 (see bf16x3_nt)
+  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
+  if (tile >= p.tiles_m * p.tiles_n) return;
+  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
+
+  uint4 ra[CA], rb[CB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < CA; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
+      const uint16_t* src = (hl ? p.Alo : p.Ahi) + (long)(m0 + m) * p.lda + k0 + ch * 8;
+      ra[it] = (m0 + m < p.M) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int it = 0; it < CB; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
+      const uint16_t* src = (hl ? p.Blo : p.Bhi) + (long)(n0 + n) * p.ldb + k0 + ch * 8;
+      rb[it] = (n0 + n < p.N) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < CA; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
+      *(uint4*)&sA[buf][hl][m * LDK + ch * 8] = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < CB; ++it) {
+      int u = it * NT + tid;
+      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
+      *(uint4*)&sB[buf][hl][n * LDK + ch * 8] = rb[it];
+    }
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+  auto compute = [&](int buf) {
+    const int kofs = 8 * (lane >> 4);
+    bf16x8_t ah[TM], al[TM];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = (wm0 + i * 16 + (lane & 15)) * LDK + kofs;
+      ah[i] = *(const bf16x8_t*)&sA[buf][0][r];
+      al[i] = *(const bf16x8_t*)&sA[buf][1][r];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = (wn0 + j * 16 + (lane & 15)) * LDK + kofs;
+      const bf16x8_t bh = *(const bf16x8_t*)&sB[buf][0][r];
+      const bf16x8_t bl = *(const bf16x8_t*)&sB[buf][1][r];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+
+  const int nk = p.Kp / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+    __syncthreads();
+  }
+  for (int kt = 0; kt < nk; ++kt) {
+    const bool more = kt + 1 < nk;
+    if (more) load((kt + 1) * BK);
+    if (DBUF) {
+      compute(kt & 1);
+      if (more) store((kt + 1) & 1);
+      __syncthreads();
+    } else {
+      compute(0);
+      __syncthreads();
+      if (more) {
+        store(0);
+        __syncthreads();
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (row < p.M && col < p.N) p.C[(long)row * p.ldc + col] = acc[i][j][r];
+      }
+}
+
+template <int BM, int BN, int WM, int WN, bool DBUF>
+hipError_t launch_big(Bf16Args p, hipStream_t s) {
+  p.tiles_m = (p.M + BM - 1) / BM;
+  p.tiles_n = (p.N + BN - 1) / BN;
+  long tiles = (long)p.tiles_m * p.tiles_n;
+  p.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
+  hipLaunchKernelGGL((bf16x3_nt_big_kernel<BM, BN, WM, WN, DBUF>), dim3(p.tiles_pad), dim3(64 * WM * WN), 0, s, p);
+  return hipGetLastError();
+}
+
+// Tile choice by modelled time: padding x wave quantisation over 256 CUs,
+// divided by the tile's relative throughput (measured on MI355X).
 hipError_t bf16x3_nt(const uint16_t* Ahi, const uint16_t* Alo, long lda, const uint16_t* Bhi, const uint16_t* Blo,
-                     long ldb, float* C, long ldc, int M, int N, int K, hipStream_t s) {
+                     long ldb, float* C, long ldc, int M, int N, int K, hipStream_t s, int force = -1) {
   Bf16Args p;
   p.Ahi = Ahi; p.Alo = Alo; p.Bhi = Bhi; p.Blo = Blo; p.C = C;
   p.lda = lda; p.ldb = ldb; p.ldc = ldc;
   p.M = M; p.N = N; p.Kp = ((K + 31) / 32) * 32;
+  struct Opt { int BM, BN; double rate; };
+  const Opt opts[3] = {{128, 128, 1.0}, {256, 256, 1.6}, {128, 512, 1.6}};
+  int best = 0;
+  double bt = 1e300;
+  for (int c = 0; c < 3; ++c) {
+    long tm = (M + opts[c].BM - 1) / opts[c].BM, tn = (N + opts[c].BN - 1) / opts[c].BN;
+    double pad = double(tm * opts[c].BM) * double(tn * opts[c].BN) / (double(M) * N);
+    double tiles = double(tm * tn);
+    double rounds = std::ceil(tiles / 256.0);
+    double t = pad * (rounds * 256.0 / tiles) / opts[c].rate;
+    if (t < bt) { bt = t; best = c; }
+  }
+  if (best == 1) best = 3;  // 256 x 256: the double-buffered variant measured ~1% faster
+  if (force >= 0 && force < 4) best = force;
+  switch (best) {
+    case 1: return launch_big<256, 256, 4, 2, false>(p, s);
+    case 2: return launch_big<128, 512, 2, 4, false>(p, s);
+    case 3: return launch_big<256, 256, 4, 2, true>(p, s);
+    default: break;
+  }
   constexpr int BM = 128, BN = 128;
   p.tiles_m = (M + BM - 1) / BM;
   p.tiles_n = (N + BN - 1) / BN;
   long tiles = (long)p.tiles_m * p.tiles_n;
-  p.tiles_pad = (int)(((tiles + 7) / 8) * 8);
+  p.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
   hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, 2, 2>), dim3(p.tiles_pad), dim3(256), 0, s, p);
   return hipGetLastError();
 }
@@ -380,6 +534,8 @@ int lowprec_ring(Ctx* c, int prec) {
 
 int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K, double alpha, double beta,
                     int prec, hipStream_t s) {
+  const int tile = (prec >> 8) & 0xff;  // 0: automatic, else forced BF16 tile configuration + 1
+  prec &= 0xff;
   const long ld = pad64(K);
   float *A32 = nullptr, *B32 = nullptr, *C32 = nullptr;
   uint16_t *Ah = nullptr, *Al = nullptr, *Bh = nullptr, *Bl = nullptr;
@@ -412,13 +568,13 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
       g.M = M; g.N = N; g.K = K;
       g.sAm = ld; g.sAk = 1; g.sBk = 1; g.sBn = ld; g.sCm = N; g.sCn = 1;
       GemmWorkspace ws{wsp, 16u << 20};
-      e = gemm<float>(g, ws, s);
+      e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
     } else {
       hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(M * ld)), dim3(kThreads), 0, s, Ah, Al, A, (long)M,
                          (long)K, (long)K, ld, 1.0);
       hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(N * ld)), dim3(kThreads), 0, s, Bh, Bl, B, (long)N,
                          (long)K, (long)K, ld, 1.0);
-      e = bf16x3_nt(Ah, Al, ld, Bh, Bl, ld, C32, N, M, N, K, s);
+      e = bf16x3_nt(Ah, Al, ld, Bh, Bl, ld, C32, N, M, N, K, s, tile - 1);
     }
   }
   if (e == hipSuccess) {

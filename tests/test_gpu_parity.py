@@ -49,6 +49,38 @@ def test_gemm_fp64(M, N, K, ta, tb):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+N_FP64_TILES = 12  # configurations of the generic MFMA engine
+N_BF16_TILES = 4   # configurations of the BF16 hi/lo kernel
+
+
+@pytest.mark.parametrize("tile", range(1, N_FP64_TILES + 1))
+@pytest.mark.parametrize("M,N,K,ta,tb", [(40, 40, 30000, 0, 1), (97, 301, 77, 1, 0), (300, 40, 300, 0, 0)])
+def test_gemm_fp64_every_tile(tile, M, N, K, ta, tb):
+    """Every tile shape, ragged edges, and the few-tile / long-K split path."""
+    capi = _ctx()
+    rng = np.random.default_rng(tile * 100 + M)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 1.3 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.4 * C0
+    out = capi.selftest_gemm(A, B, C0, 1.3, 0.4, ta=bool(ta), tb=bool(tb), precision="fp64", tile=tile)
+    assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))) * np.sqrt(K / 1000 + 1)
+
+
+@pytest.mark.parametrize("prec,tile,tol", [("fp32", t, 2e-5) for t in (1, 9, 10, 12)] +
+                         [("bf16x3", t, 5e-5) for t in range(1, N_BF16_TILES + 1)])
+@pytest.mark.parametrize("M,N,K", [(300, 520, 200), (40, 45, 2000)])
+def test_gemm_lowprec_every_tile(prec, tile, tol, M, N, K):
+    capi = _ctx()
+    rng = np.random.default_rng(M + N + K + tile)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = A @ B - 0.5 * C0
+    out = capi.selftest_gemm(A, B, C0, 1.0, -0.5, precision=prec, tile=tile)
+    assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 5e-5)])
 @pytest.mark.parametrize("M,N,K", [(130, 70, 300), (256, 256, 1024), (33, 129, 65)])
 def test_gemm_lowprec(prec, tol, M, N, K):
