@@ -2,7 +2,9 @@
 
 Every source under ``csrc/`` is compiled with hipcc (device code for gfx950
 only) and linked into one shared library exporting the C ABI declared in
-``include/ccsd_amd.h`` and the drop-in class of ``include/ccsd_memory_optimized.h``.
+``include/ccsd_amd.h`` and ``include/ccsd_rhf.h`` and the drop-in class of
+``include/ccsd_memory_optimized.h``; the host programs under ``apps/`` (the
+INCAR front-end) are linked against it with g++.
 Run ``python -m ccsd_parallel_amd.build`` or call :func:`build`.
 """
 import concurrent.futures as cf
@@ -14,6 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+APPS = os.path.join(HERE, "apps")
 OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libccsd_amd.so")
@@ -70,7 +73,23 @@ def build(verbose=True, jobs=None):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_apps(verbose)
     return LIB
+
+
+def build_apps(verbose=True):
+    """Host programs under ``apps/`` (g++, linked against the library, rpath to it)."""
+    for f in sorted(os.listdir(APPS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(LIBDIR, f[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(APPS, f),
+               "-L" + LIBDIR, "-lccsd_amd", "-Wl,-rpath,$ORIGIN", "-o", exe]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"g++ failed for {f}:\n" + p.stderr)
+        if verbose:
+            print("built", exe)
 
 
 def clean():

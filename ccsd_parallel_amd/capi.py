@@ -24,7 +24,11 @@ EXPORTED = (
     "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
     "ccsd_selftest_gemm", "ccsd_update_integrals",
+    # include/ccsd_rhf.h (host RHF input producer)
+    "ccsd_basis_size", "ccsd_rhf", "ccsd_ao_one_electron",
 )
+
+ANGSTROM_TO_BOHR = 1.889725989  # the reference's factor (initializer.cpp:56-59)
 
 
 class Options(C.Structure):
@@ -54,6 +58,11 @@ class Result(C.Structure):
 class GradInfo(C.Structure):
     _fields_ = [("lambda_iterations", C.c_int), ("converged", C.c_int), ("lambda_residual", C.c_double),
                 ("seconds", C.c_double)]
+
+
+class RhfResult(C.Structure):
+    _fields_ = [("e_hf", C.c_double), ("e_nuc", C.c_double), ("commutator", C.c_double), ("n_basis", C.c_int),
+                ("n_occ", C.c_int), ("iterations", C.c_int), ("converged", C.c_int)]
 
 
 class CCSDError(RuntimeError):
@@ -98,6 +107,11 @@ def load():
     lib.ccsd_update_integrals.argtypes = [vp, C.c_int, C.c_long, C.POINTER(C.c_long), dp]
     lib.ccsd_selftest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_double,
                                        dp, C.c_int]
+    ip = C.POINTER(C.c_int)
+    lib.ccsd_basis_size.argtypes = [C.c_int, ip, C.c_char_p]
+    lib.ccsd_rhf.argtypes = [C.c_int, ip, dp, C.c_char_p, C.c_int, C.c_double, C.POINTER(RhfResult), dp, dp, dp, dp,
+                             C.c_int]
+    lib.ccsd_ao_one_electron.argtypes = [C.c_int, ip, dp, C.c_char_p, dp, dp, dp, C.c_int]
     _lib = lib
     return lib
 
@@ -280,3 +294,50 @@ def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision
                                      PRECISIONS.get(precision, precision) | (int(tile) << 8)),
            "ccsd_selftest_gemm")
     return Cm
+
+
+def _molecule(Z, R, angstrom):
+    Zc = np.ascontiguousarray(Z, dtype=np.int32)
+    Rc = np.ascontiguousarray(R, dtype=np.float64).reshape(len(Zc), 3)
+    if angstrom:
+        Rc = Rc * ANGSTROM_TO_BOHR
+    return Zc, np.ascontiguousarray(Rc)
+
+
+def basis_size(Z, basis):
+    Zc = np.ascontiguousarray(Z, dtype=np.int32)
+    n = load().ccsd_basis_size(len(Zc), Zc.ctypes.data_as(C.POINTER(C.c_int)), basis.encode())
+    _check(min(n, 0), "ccsd_basis_size")
+    return n
+
+
+def rhf(Z, R, basis, n_electrons=None, tol=1e-10, angstrom=False, want_ao_eri=False, allow_unconverged=False):
+    """Closed-shell RHF (host code in the library): returns a dict with e_hf,
+    e_nuc, eps, C (C[mu, p]), mo_eri ((pq|rs), MO basis) and, if asked, ao_eri."""
+    Zc, Rc = _molecule(Z, R, angstrom)
+    n = basis_size(Zc, basis)
+    ne = int(Zc.sum()) if n_electrons is None else int(n_electrons)
+    res = RhfResult()
+    Cm = np.zeros((n, n))
+    eps = np.zeros(n)
+    mo = np.zeros((n, n, n, n))
+    ao = np.zeros((n, n, n, n)) if want_ao_eri else None
+    rc = load().ccsd_rhf(len(Zc), Zc.ctypes.data_as(C.POINTER(C.c_int)), _ptr(Rc), basis.encode(), ne, tol,
+                         C.byref(res), _ptr(Cm), _ptr(eps), _ptr(mo), _ptr(ao) if ao is not None else None, n)
+    if not (rc == 0 or (allow_unconverged and rc == -5)):
+        _check(rc, "ccsd_rhf")
+    out = {k: getattr(res, k) for k, _ in RhfResult._fields_}
+    out.update(C=Cm, eps=eps, mo_eri=mo, nocc=res.n_occ)
+    if ao is not None:
+        out["ao_eri"] = ao
+    return out
+
+
+def ao_one_electron(Z, R, basis, angstrom=False):
+    """AO overlap, kinetic and nuclear-attraction matrices."""
+    Zc, Rc = _molecule(Z, R, angstrom)
+    n = basis_size(Zc, basis)
+    S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    _check(load().ccsd_ao_one_electron(len(Zc), Zc.ctypes.data_as(C.POINTER(C.c_int)), _ptr(Rc), basis.encode(),
+                                       _ptr(S), _ptr(T), _ptr(V), n), "ccsd_ao_one_electron")
+    return S, T, V

@@ -158,7 +158,34 @@ double CCSD_Memory_optimized::CCSD(double toler, bool print_stuff) {
     fprintf(stderr, "CCSD_Memory_optimized: %s\n", ccsd_strerror(status_));
     return std::nan("");
   }
+  const size_t n1 = (size_t)O * V, n2 = (size_t)p2(O) * p2(V);
+  restarted_ = false;
+  if (!checkpoint_.empty()) {
+    if (FILE* fp = fopen(checkpoint_.c_str(), "rb")) {
+      int hdr[2] = {0, 0};
+      std::vector<double> t1(n1), t2(n2);
+      if (fread(hdr, 4, 2, fp) == 2 && hdr[0] == O && hdr[1] == V && fread(t1.data(), 8, n1, fp) == n1 &&
+          fread(t2.data(), 8, n2, fp) == n2 && ccsd_set_amplitudes(ctx, t1.data(), t2.data()) == CCSD_OK)
+        restarted_ = true;
+      fclose(fp);
+      if (print_stuff) printf("Checkpoint %s: %s\n", checkpoint_.c_str(), restarted_ ? "restart" : "not used");
+    }
+  }
   status_ = ccsd_solve(ctx, toler, 1000, &result_);  // Itermax = 1000 (:42)
+  if (!checkpoint_.empty() && (status_ == CCSD_OK || status_ == CCSD_ERR_NOT_CONVERGED)) {
+    std::vector<double> t1(n1), t2(n2);
+    if (ccsd_get_amplitudes(ctx, t1.data(), t2.data()) == CCSD_OK) {
+      const std::string tmp = checkpoint_ + ".tmp";
+      if (FILE* fp = fopen(tmp.c_str(), "wb")) {
+        int hdr[2] = {O, V};
+        bool ok = fwrite(hdr, 4, 2, fp) == 2 && fwrite(t1.data(), 8, n1, fp) == n1 &&
+                  fwrite(t2.data(), 8, n2, fp) == n2;
+        ok = (fclose(fp) == 0) && ok;
+        if (ok) std::rename(tmp.c_str(), checkpoint_.c_str());
+        else std::remove(tmp.c_str());
+      }
+    }
+  }
   ccsd_destroy(ctx);
   if (status_ != CCSD_OK && status_ != CCSD_ERR_NOT_CONVERGED) return std::nan("");
   e_corr_ = result_.e_corr;

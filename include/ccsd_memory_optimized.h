@@ -84,6 +84,12 @@ class CCSD_Memory_optimized {
   const ccsd_result& last_result() const { return result_; }
   int last_status() const { return status_; }
   void set_options(const ccsd_options& o) { opt_ = o; have_opt_ = true; }
+  // Amplitude checkpoint: when the file exists and holds amplitudes of the
+  // same shape, CCSD() starts from them instead of the MP2 guess; after the
+  // iteration the final amplitudes are written back.  Format (little endian):
+  // int32 O, V; double t1[O*V]; double t2[P2(O)*P2(V)] (the C ABI's layouts).
+  void set_checkpoint(const std::string& path) { checkpoint_ = path; }
+  bool restarted() const { return restarted_; }
   int nuclei() const { return n_Nuclei; }
   int mpi_rank() const { return rank; }  // kept for the reference's signature; one process here
   int mpi_size() const { return size; }
@@ -104,6 +110,8 @@ class CCSD_Memory_optimized {
   double e_hf_ = 0.0, e_corr_ = 0.0;
   ccsd_options opt_{};
   bool have_opt_ = false;
+  std::string checkpoint_;
+  bool restarted_ = false;
   ccsd_result result_{};
   int status_ = 0;
   // spin-orbital packed blocks handed to the C ABI

@@ -78,3 +78,55 @@ def test_dropin_class_matches_oracle(tmp_path):
     assert abs(float(vals["provider_corr"]) - E) < 1e-10
     assert abs(float(vals["direct_corr"]) - E) < 1e-10
     assert abs(float(vals["direct_total"]) - (E + ehf)) < 1e-10
+
+
+# -- config 1: H2O / 6-31G through the INCAR front-end and the drop-in class
+
+def _incar_run(tmp_path, name, method="CCSD", extra="", basis="6-31G", angstrom=True, atoms=None, toler=-10.0):
+    from tests.test_rhf import APP, WATER_ANG
+    atoms = atoms or [("O", WATER_ANG[0]), ("H", WATER_ANG[1]), ("H", WATER_ANG[2])]
+    lines = [f"Basis_Set {basis}", f"Method {method}", f"convergance_criteria {toler}", "Relax_Pos false",
+             f"use_angstrom {'true' if angstrom else 'false'}", "print_stuffies false", extra, "", "#ATOMS START"]
+    lines += [f"{s} {x} {y} {z}" for s, (x, y, z) in atoms] + ["#ATOMS END", ""]
+    p = tmp_path / name
+    p.write_text("\n".join(lines))
+    print("case:", name, extra, flush=True)
+    out = subprocess.run([APP, str(p)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    last = out.stdout.strip().splitlines()[-1].split()
+    return dict(zip(last[0::2], (float(v) for v in last[1::2])))
+
+
+@pytest.mark.gpu
+def test_config1_h2o_631g_matches_oracle(tmp_path):
+    """RHF -> CCSD through the reference's entry point on H2O / 6-31G (13 basis
+    functions, 5 doubly occupied): the GPU FP64 E_corr equals the oracle's on
+    the same MO integrals to 1e-8 Eh (the bar; observed ~1e-12)."""
+    from ccsd_parallel_amd import capi
+    from tests.test_rhf import WATER_ANG, _oracle_ccsd
+    res = _incar_run(tmp_path, "INCAR_c1")
+    r = capi.rhf([8, 1, 1], WATER_ANG, "6-31G", angstrom=True)
+    e_ref = _oracle_ccsd(r, toler=1e-12)
+    assert abs(res["E_HF"] - r["e_hf"]) < 1e-10
+    assert abs(res["E_corr"] - e_ref) < 1e-8
+    assert abs(res["E_total"] - (r["e_hf"] + e_ref)) < 1e-8
+
+
+@pytest.mark.gpu
+def test_reference_sto3g_comment_value_on_gpu(tmp_path):
+    """H2O / STO-3G at the INCAR~ geometry (bohr): the reference's printed -0.0501273."""
+    atoms = [("O", (0, 0, 0)), ("H", (0, 1.079252, 1.474611)), ("H", (0, 1.079252, -1.474611))]
+    res = _incar_run(tmp_path, "INCAR_kat", basis="STO-3G", angstrom=False, atoms=atoms)
+    assert abs(res["E_corr"] - (-0.0501273)) < 5e-8
+
+
+@pytest.mark.gpu
+def test_incar_checkpoint_restart_and_mixed_precision(tmp_path):
+    ck = tmp_path / "amps.bin"
+    a = _incar_run(tmp_path, "INCAR_a", extra=f"Checkpoint {ck}")
+    assert ck.exists() and a["restarted"] == 0
+    b = _incar_run(tmp_path, "INCAR_b", extra=f"Checkpoint {ck}")
+    assert b["restarted"] == 1 and b["iterations"] <= 3
+    assert abs(a["E_corr"] - b["E_corr"]) < 1e-10
+    m = _incar_run(tmp_path, "INCAR_m", extra="Precision bf16x3")
+    assert abs(m["E_corr"] - a["E_corr"]) < 1e-7

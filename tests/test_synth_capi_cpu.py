@@ -61,7 +61,9 @@ def test_generator_statistics():
 # ----------------------------------------------------------------- C ABI surface
 
 def _declared_symbols():
-    src = open(HEADER).read()
+    """Every C-ABI function declared in include/*.h."""
+    inc = os.path.dirname(HEADER)
+    src = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     decl = r"^\s*(?:int|void|size_t|long|const char\s*\*)\s+(ccsd_[a-z_0-9]+)\s*\("
     return sorted(set(re.findall(decl, src, flags=re.M)))
 
