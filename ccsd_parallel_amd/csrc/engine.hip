@@ -197,10 +197,24 @@ int setup_synthetic(Ctx* c, uint64_t seed, double scale) {
   return finish_static(c, c->X);
 }
 
+// FP32 product into an FP64 destination (the low-precision iterations).
+static int run_f32(Ctx* c, const GemmDesc<float>& g) {
+  hipError_t e = gemm<float>(g, c->ws, c->stream);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  return CCSD_OK;
+}
+
 // All right-hand-side pieces at amplitudes (t1, t2) (device pointers):
 //   Rp  = <ab||ij> + pp ladder + hh ladder           (packed)
 //   X   = ring GEMM (P(ij)P(ab) applied at assembly)
 //   Y   = P(ab) pieces, Z = P(ij) pieces, R1 = T1 right-hand side.
+// FP64 throughout when prec is FP64;
+// otherwise the four dominant contractions run in `prec` (lowprec.hip) and all
+// other contractions on FP32 copies of their operands, each accumulating into
+// the same FP64 intermediates.
 int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_it) {
   const Dims& d = c->d;
   const long O = d.O, V = d.V, VV = V * V, po = d.po, pv = d.pv, ov = d.ov;
@@ -208,32 +222,60 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   const bool lowp = prec != CCSD_PREC_FP64;
   float ms_pp = 0, ms_ring = 0, ms_wq = 0, ms_w2 = 0, ms_all = 0;
   if (time_it) HIPTRY(hipEventRecord(c->ev[0], s));
+  LowPrec* L = nullptr;
+  // contraction g in FP64, or on the FP32 operands A32 / B32 (g's strides)
+  auto mm = [&](const GemmDesc<double>& g, const float* A32, const float* B32) -> int {
+    return lowp ? run_f32(c, f32_desc(g, A32, B32)) : run_gemm(c, g);
+  };
 
   // -- amplitude layouts
   k_build_tau(c->tau, t2, t1, d, s);
-  k_unpack_occ(c->T2u, c->tauu, t2, c->tau, d, s);
-  k_unpack_vir(c->Tv, t2, d, s);
-  k_unpack_ring(c->Tring, t2, d, s);
-  if (lowp) TRY(lowprec_amplitudes(c, prec));
+  if (lowp) {
+    TRY(lowprec_allocate(c, prec));
+    L = c->lp;
+    k_unpack_occ_f32(L->T2u_f, L->tauu_f, t2, c->tau, d, s);
+    k_unpack_vir_ring_lp(L->Tv_f, lowprec_ring_out(c, prec), t2, d, s);
+    k_to_f32(L->t1_f, t1, ov, s);
+    TRY(lowprec_amplitudes(c, prec));
+  } else {
+    k_unpack_occ(c->T2u, c->tauu, t2, c->tau, d, s);
+    k_unpack_vir_ring(c->Tv, c->Tring, t2, d, s);
+  }
+  const float* t1f = lowp ? L->t1_f : nullptr;
 
   // -- F_me = f_me + <mn||ef> t_n^f
   k_fme_init(c->Fme, c->f, d, s);
-  TRY(run_gemm(c, G64(c->Doovv, ov, 1, t1, 1, 0, c->Fme, 1, 0, ov, 1, ov, 1.0, 1.0)));
+  if (lowp) TRY(lowprec_fme(c, prec, t1));
+  else TRY(run_gemm(c, G64(c->Doovv, ov, 1, t1, 1, 0, c->Fme, 1, 0, ov, 1, ov, 1.0, 1.0)));
 
   // -- W_mnij = <mn||ij> + 1/2 tau <mn||ef> + P(ij) t_j^e <mn||ie>
   k_copy(c->W1, c->oooo, po * po, s);
-  TRY(run_gemm(c, G64(c->oovv, pv, 1, c->tau, 1, pv, c->W1, po, 1, po, po, pv, 1.0, 1.0)));
+  {
+    GemmDesc<double> g = G64(c->oovv, pv, 1, c->tau, 1, pv, c->W1, po, 1, po, po, pv, 1.0, 1.0);
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->oovv_f, L->tau32);
+      f.sBn = L->ld_pv;
+      TRY(run_f32(c, f));
+    } else {
+      TRY(run_gemm(c, g));
+    }
+  }
   TRY(run_gemm(c, G64(c->ooov, V, 1, t1, 1, V, c->Q, O, 1, po * O, O, V, 1.0, 0.0)));
   k_w1_finish(c->W1, c->Q, d, s);
+  if (lowp) k_to_f32(L->W1_f, c->W1, po * po, s);
 
   // -- W_mnie (layout [i][mn][e]) = <mn||ie> + t_i^f <mn||fe>
   k_w3_init(c->W3, c->ooov, d, s);
   {
     GemmDesc<double> g = G64(t1, V, 1, c->oovv_vu, V, 1, c->W3, po * V, 1, O, V, V, 1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = VV; g.sC1 = V;
-    TRY(run_gemm(c, g));
+    TRY(mm(g, t1f, lowp ? L->oovv_vu_f : nullptr));
   }
   k_w3_unpack(c->W3u, c->W3, d, s);
+  if (lowp) {
+    k_to_f32(L->W3_f, c->W3, O * po * V, s);
+    k_to_f32(L->W3u_f, c->W3u, O * O * O * V, s);
+  }
 
   // -- W_mbij (layout [ij][(m,b)]) = <mb||ij> + 1/2 tau <mb||ef> + P(ij) t_i^e <mb||ej>
   k_copy(c->W2, c->ooov, po * ov, s);
@@ -249,18 +291,30 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = m, b2 = j
     g.sB1 = V * ov; g.sB2 = V;
     g.sC1 = V; g.sC2 = ov;
-    TRY(run_gemm(c, g));
+    TRY(mm(g, t1f, lowp ? L->G_f : nullptr));
   }
   k_w2_finish(c->W2, c->U, d, s);
+  if (lowp) k_to_f32(L->W2_f, c->W2, po * ov, s);
 
   // -- W_mbej stored transposed: WT[(j,b),(m,e)]
   {
     // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f as one tall GEMM (M = O V^2), staged in X
     // (free until the ring GEMM); then WT = S^(permuted) + <mb||ej>.
     double* S = c->X;
-    TRY(run_gemm(c, G64(c->ovvv_u, V, 1, t1, 1, V, S, O, 1, O * VV, O, V, 1.0, 0.0)));
-    k_fae_t1_from_s(c->Fae_t1, S, d, s);  // F_ae t1 part = -sum_m S[(m,a,e)][m]
-    k_wt_from_s(c->WT, S, c->G, d, s);
+    GemmDesc<double> g = G64(c->ovvv_u, V, 1, t1, 1, V, S, O, 1, O * VV, O, V, 1.0, 0.0);
+    if (lowp) {  // S in FP32 (half the traffic of the two consumers)
+      float* S32 = reinterpret_cast<float*>(c->X);
+      GemmDesc<float> f = f32_desc(g, L->ovvv_u_f, t1f);
+      f.Cd = nullptr;
+      f.C = S32;
+      TRY(run_f32(c, f));
+      k_fae_t1_from_s_f32(c->Fae_t1, S32, d, s);
+      k_wt_from_s_f32(c->WT, S32, L->G_f, d, s);
+    } else {
+      TRY(run_gemm(c, g));
+      k_fae_t1_from_s(c->Fae_t1, S, d, s);  // F_ae t1 part = -sum_m S[(m,a,e)][m]
+      k_wt_from_s(c->WT, S, c->G, d, s);
+    }
   }
   {
     // + t_n^b W_mnje, batched over (j, m)
@@ -268,7 +322,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = j, b2 = m
     g.sB1 = O * O * V; g.sB2 = O * V;
     g.sC1 = V * ov; g.sC2 = V;
-    TRY(run_gemm(c, g));
+    TRY(mm(g, t1f, lowp ? L->W3u_f : nullptr));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[3], s));
   if (lowp) {
@@ -281,11 +335,15 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
 
   // -- F_mi
   k_fmi_init(c->Fmi, c->f, c->ooov, t1, d, s);
-  TRY(run_gemm(c, G64(c->oovv_ou, O * pv, 1, c->T2u, 1, O * pv, c->Fmi, O, 1, O, O, O * pv, 1.0, 1.0)));
+  TRY(mm(G64(c->oovv_ou, O * pv, 1, c->T2u, 1, O * pv, c->Fmi, O, 1, O, O, O * pv, 1.0, 1.0),
+         lowp ? L->oovv_ou_f : nullptr, lowp ? L->T2u_f : nullptr));
   k_fmi_finish(c->Fmi, c->Fme, t1, d, s);
+  if (lowp) k_to_f32(L->Fmi_f, c->Fmi, O * O, s);
   // -- F_ae
   k_fae_init(c->Fae, c->f, c->Fae_t1, c->Fme, t1, d, s);
-  TRY(run_gemm(c, G64(c->Tv, 1, V, c->oovv_vu, V, 1, c->Fae, V, 1, V, V, po * V, -1.0, 1.0)));
+  TRY(mm(G64(c->Tv, 1, V, c->oovv_vu, V, 1, c->Fae, V, 1, V, V, po * V, -1.0, 1.0), lowp ? L->Tv_f : nullptr,
+         lowp ? L->oovv_vu_f : nullptr));
+  if (lowp) k_to_f32(L->Fae_f, c->Fae, VV, s);
 
   // -- packed residual: <ab||ij> + pp ladder + hh ladder
   k_copy(c->Rp, c->oovv, po * pv, s);
@@ -296,7 +354,16 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, pv, 1, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0)));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[6], s));
-  TRY(run_gemm(c, G64(c->W1, 1, po, c->tau, pv, 1, c->Rp, pv, 1, po, pv, po, 1.0, 1.0)));
+  {
+    GemmDesc<double> g = G64(c->W1, 1, po, c->tau, pv, 1, c->Rp, pv, 1, po, pv, po, 1.0, 1.0);
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->W1_f, L->tau32);
+      f.sBk = L->ld_pv;
+      TRY(run_f32(c, f));
+    } else {
+      TRY(run_gemm(c, g));
+    }
+  }
 
   // -- ring X = Tring * W_mbej
   if (time_it) HIPTRY(hipEventRecord(c->ev[7], s));
@@ -308,22 +375,29 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (time_it) HIPTRY(hipEventRecord(c->ev[8], s));
 
   // -- Y[ij][a][b] = t_ij^ae F_be - t_m^a W_mbij
-  TRY(run_gemm(c, G64(c->Tv, V, 1, c->Fae, 1, V, c->Y, V, 1, po * V, V, V, 1.0, 0.0)));
+  TRY(mm(G64(c->Tv, V, 1, c->Fae, 1, V, c->Y, V, 1, po * V, V, V, 1.0, 0.0), lowp ? L->Tv_f : nullptr,
+         lowp ? L->Fae_f : nullptr));
   {
     GemmDesc<double> g = G64(t1, 1, V, c->W2, V, 1, c->Y, V, 1, V, V, O, -1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = ov; g.sC1 = VV;
-    TRY(run_gemm(c, g));
+    TRY(mm(g, t1f, lowp ? L->W2_f : nullptr));
   }
   // -- Z[i][j][ab] = -t_im^ab F_mj - t_i^e <je||ab>
   {
     GemmDesc<double> g = G64(c->Fmi, 1, O, c->T2u, pv, 1, c->Z, pv, 1, O, pv, O, -1.0, 0.0);
     g.nb1 = (int)O; g.sB1 = O * pv; g.sC1 = O * pv;
-    TRY(run_gemm(c, g));
+    TRY(mm(g, lowp ? L->Fmi_f : nullptr, lowp ? L->T2u_f : nullptr));
   }
   {
     GemmDesc<double> g = G64(t1, V, 1, c->ovvv, pv, 1, c->Z, O * pv, 1, O, pv, V, -1.0, 1.0);
     g.nb1 = (int)O; g.sB1 = V * pv; g.sC1 = pv;
-    TRY(run_gemm(c, g));
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, t1f, L->ovvv32);
+      f.sBk = L->ld_pv; f.sB1 = V * L->ld_pv;
+      TRY(run_f32(c, f));
+    } else {
+      TRY(run_gemm(c, g));
+    }
   }
 
   // -- T1 right-hand side
@@ -332,11 +406,19 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     // -1/2 tau_im^ef <ma||ef>, summed over m
     GemmDesc<double> g = G64(c->tauu, O * pv, 1, c->ovvv, 1, pv, c->R1, V, 1, O, V, pv, -1.0, 1.0);
     g.nb1 = (int)O; g.sA1 = pv; g.sB1 = V * pv; g.reduce_batch = true;
-    TRY(run_gemm(c, g));
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->tauu_f, L->ovvv32);
+      f.sBn = L->ld_pv; f.sB1 = V * L->ld_pv;
+      TRY(run_f32(c, f));
+    } else {
+      TRY(run_gemm(c, g));
+    }
   }
-  TRY(run_gemm(c, G64(c->W3, po * V, 1, c->Tv, V, 1, c->R1, V, 1, O, V, po * V, 1.0, 1.0)));  // -1/2 t_mn^ae W_mnie
-  TRY(run_gemm(c, G64(c->Tring, ov, 1, c->Fme, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0)));       // t_im^ae F_me
-  TRY(run_gemm(c, G64(c->G, ov, 1, t1, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0)));               // -t_n^f <na||if>
+  TRY(mm(G64(c->W3, po * V, 1, c->Tv, V, 1, c->R1, V, 1, O, V, po * V, 1.0, 1.0), lowp ? L->W3_f : nullptr,
+         lowp ? L->Tv_f : nullptr));                                                                   // -1/2 t_mn^ae W_mnie
+  if (lowp) TRY(lowprec_r1_ring(c, prec));                                                       // t_im^ae F_me
+  else TRY(run_gemm(c, G64(c->Tring, ov, 1, c->Fme, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0)));
+  TRY(mm(G64(c->G, ov, 1, t1, 1, 0, c->R1, 1, 0, ov, 1, ov, 1.0, 1.0), lowp ? L->G_f : nullptr, t1f));  // -t_n^f <na||if>
   TRY(run_gemm(c, G64(c->Fmi, 1, O, t1, V, 1, c->R1, V, 1, O, V, O, -1.0, 1.0)));               // -t_m^a F_mi
 
   if (time_it) {

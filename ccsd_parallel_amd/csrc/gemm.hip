@@ -42,6 +42,20 @@ struct Mfma<float> {
   static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// C[off] = alpha * s + beta * C[off], into the FP64 destination Cd when given.
+template <typename T>
+__device__ __forceinline__ void write_out(const GemmDesc<T>& d, long off, T s) {
+  if (d.Cd) {
+    double v = (double)d.alpha * (double)s;
+    if (d.beta != T(0)) v += (double)d.beta * d.Cd[off];
+    d.Cd[off] = v;
+  } else {
+    T v = d.alpha * s;
+    if (d.beta != T(0)) v += d.beta * d.C[off];
+    d.C[off] = v;
+  }
+}
+
 template <typename T>
 struct KArgs {
   GemmDesc<T> d;
@@ -224,8 +238,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
           if (row < d.M && col < d.N) W[(long)row * d.N + col] = acc[i][j][r];
         }
   } else {
-    T* C = d.C + b1 * d.sC1 + b2 * d.sC2;
-    const T alpha = d.alpha, beta = d.beta;
+    const long cbase = b1 * d.sC1 + b2 * d.sC2;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -234,12 +247,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
         for (int r = 0; r < 4; ++r) {
           int row = m0 + wm0 + i * 16 + Mfma<T>::out_row(lane, r);
           int col = n0 + wn0 + j * 16 + (lane & 15);
-          if (row < d.M && col < d.N) {
-            T* c = C + row * d.sCm + col * d.sCn;
-            T v = alpha * acc[i][j][r];
-            if (beta != T(0)) v += beta * (*c);
-            *c = v;
-          }
+          if (row < d.M && col < d.N) write_out(d, cbase + row * d.sCm + col * d.sCn, (T)acc[i][j][r]);
         }
   }
 }
@@ -269,9 +277,7 @@ __global__ void gemm_reduce_wide_kernel(GemmDesc<T> d, const T* __restrict__ W, 
     const int row = (int)(e / d.N), col = (int)(e % d.N);
     long off = row * d.sCm + col * d.sCn;
     if (!d.reduce_batch) off += (bo / d.nb2) * d.sC1 + (bo % d.nb2) * d.sC2;
-    T v = d.alpha * sh[0];
-    if (d.beta != T(0)) v += d.beta * d.C[off];
-    d.C[off] = v;
+    write_out(d, off, sh[0]);
   }
 }
 
@@ -297,9 +303,7 @@ __global__ void gemm_reduce_kernel(GemmDesc<T> d, const T* __restrict__ W, int s
     }
     long off = row * d.sCm + col * d.sCn;
     if (!d.reduce_batch) off += (bo / d.nb2) * d.sC1 + (bo % d.nb2) * d.sC2;
-    T v = d.alpha * s;
-    if (d.beta != T(0)) v += d.beta * d.C[off];
-    d.C[off] = v;
+    write_out(d, off, s);
   }
 }
 
@@ -314,12 +318,7 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
   T s = 0;
   for (int k = lane; k < d.K; k += 64) s += a[k * d.sAk] * d.B[k * d.sBk];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) {
-    T* c = d.C + (long)row * d.sCm;
-    T v = d.alpha * s;
-    if (d.beta != T(0)) v += d.beta * (*c);
-    *c = v;
-  }
+  if (lane == 0) write_out(d, (long)row * d.sCm, s);
 }
 
 // rate: FP64 TF/s of the config on a 12000^3 GEMM on MI355X (tile-sweep probe);

@@ -27,6 +27,10 @@ struct GemmDesc {
   long sA1 = 0, sA2 = 0, sB1 = 0, sB2 = 0, sC1 = 0, sC2 = 0;
   T alpha = 1, beta = 0;
   bool reduce_batch = false;
+  // When set, the result goes to this FP64 array (same offsets as C, which is
+  // then unused): C_d = alpha * (A B accumulated in T) + beta * C_d.  Lets a
+  // lower-precision product update an FP64 intermediate without a second pass.
+  double* Cd = nullptr;
 };
 
 // Scratch the engine may use for split-K / batch-reduce partials.

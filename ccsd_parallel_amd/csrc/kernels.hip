@@ -1,6 +1,7 @@
 // Bandwidth-bound CCSD kernels (FP64).  All are grid-stride loops over the
 // destination so every write is coalesced; gathers from the packed
 // antisymmetric storage carry the permutation sign.
+#include "bf16.hpp"
 #include "kernels.hpp"
 #include <algorithm>
 
@@ -130,7 +131,8 @@ __global__ void build_tau_kernel(double* tau, const double* t2, const double* t1
   }
 }
 
-__global__ void unpack_occ_kernel(double* T2u, double* tauu, const double* t2, const double* tau, Dims d) {
+template <typename T>
+__global__ void unpack_occ_kernel(T* T2u, T* tauu, const double* t2, const double* tau, Dims d) {
   const long n = (long)d.O * d.O * d.pv;
   GRID_LOOP(idx, n) {
     long im = idx / d.pv, ab = idx % d.pv;
@@ -139,31 +141,70 @@ __global__ void unpack_occ_kernel(double* T2u, double* tauu, const double* t2, c
     double s = sgn_pair(i, m, k);
     double v1 = 0.0, v2 = 0.0;
     if (s != 0.0) { v1 = s * t2[k * d.pv + ab]; v2 = s * tau[k * d.pv + ab]; }
-    T2u[idx] = v1;
-    tauu[idx] = v2;
+    T2u[idx] = (T)v1;
+    tauu[idx] = (T)v2;
   }
 }
 
-__global__ void unpack_vir_kernel(double* Tv, const double* t2, Dims d) {
-  const long VV = (long)d.V * d.V;
-  const long n = d.po * VV;
-  GRID_LOOP(idx, n) {
-    long ij = idx / VV, ab = idx % VV;
-    int a = (int)(ab / d.V), b = (int)(ab % d.V);
-    long k;
-    double s = sgn_pair(a, b, k);
-    Tv[idx] = (s == 0.0) ? 0.0 : s * t2[ij * d.pv + k];
+// Virtual-pair and ring unpacks of t2, tiled through LDS.  Block (k1, a-tile):
+// for the occupied pair k1 = (i<m) the V x V antisymmetric matrix
+// t(a, e) = t_im^ae is gathered for kUnpackA rows a (each packed element read
+// once, in 128-byte runs), then written as full contiguous rows of
+//   Tv[k1][a][e] = t(a, e),  Tr[(i,a),(m,e)] = t(a, e),  Tr[(m,a),(i,e)] = -t(a, e).
+// Blocks with k1 >= P2(O) write the zero diagonal blocks Tr[(i,a),(i,e)].
+constexpr int kUnpackA = 16;
+__device__ __forceinline__ void ring_store(const RingOut& o, long row, long col, double v) {
+  const long off = row * o.ld + col;
+  if (o.d) {
+    o.d[off] = v;
+  } else if (o.f) {
+    o.f[off] = (float)v;
+  } else {
+    uint16_t h, l;
+    bf16_split(v, h, l);
+    o.h[off] = h;
+    o.l[off] = l;
   }
 }
 
-__global__ void unpack_ring_kernel(double* Tr, const double* t2, Dims d) {
-  const long n = d.ov * d.ov;
-  GRID_LOOP(idx, n) {
-    long r = idx / d.ov, c = idx % d.ov;  // r = (i,a), c = (m,e)
-    int i = (int)(r / d.V), a = (int)(r % d.V), m = (int)(c / d.V), e = (int)(c % d.V);
-    long k1, k2;
-    double s = sgn_pair(i, m, k1) * sgn_pair(a, e, k2);
-    Tr[idx] = (s == 0.0) ? 0.0 : s * t2[k1 * d.pv + k2];
+template <typename TV>
+__global__ __launch_bounds__(256) void unpack_pairs_kernel(TV* Tv, RingOut Tr, const double* __restrict__ t2,
+                                                           Dims d) {
+  extern __shared__ double tile[];  // kUnpackA x (V + 1)
+  const int V = (int)d.V, LD = V + 1;
+  const long k1 = blockIdx.x;
+  const bool ring = Tr.d || Tr.f || Tr.h;
+  const int a0 = blockIdx.y * kUnpackA;
+  const int na = min(kUnpackA, V - a0);
+  if (k1 >= d.po) {
+    if (!ring) return;
+    const long i = k1 - d.po;
+    for (int idx = threadIdx.x; idx < na * V; idx += blockDim.x) {
+      const int la = idx / V, e = idx % V;
+      ring_store(Tr, i * V + a0 + la, i * V + e, 0.0);
+    }
+    return;
+  }
+  const double* row = t2 + k1 * d.pv;
+  for (int idx = threadIdx.x; idx < na * V; idx += blockDim.x) {
+    const int e = idx / na, la = idx % na, a = a0 + la;
+    double v = 0.0;
+    if (e > a) v = row[pidx(a, e)];
+    else if (e < a) v = -row[pidx(e, a)];
+    tile[la * LD + e] = v;
+  }
+  __syncthreads();
+  int i, m;
+  pair_decode(k1, i, m);
+  for (int idx = threadIdx.x; idx < na * V; idx += blockDim.x) {
+    const int la = idx / V, e = idx % V;
+    const long a = a0 + la;
+    const double v = tile[la * LD + e];
+    if (Tv) Tv[(k1 * V + a) * V + e] = (TV)v;
+    if (ring) {
+      ring_store(Tr, (long)i * V + a, (long)m * V + e, v);
+      ring_store(Tr, (long)m * V + a, (long)i * V + e, -v);
+    }
   }
 }
 
@@ -229,34 +270,36 @@ __global__ void w2_finish_kernel(double* W2, const double* U, Dims d) {
 // WT[(j,b),(m,e)] = S[(m,b,e)][j] + G[(j,b),(m,e)], S = sum_f <mb||ef> t_j^f.
 // One workgroup per (m,b) and 64-wide chunk of e: the [e][j] block is staged in
 // LDS so that both the read of S and the write of WT are coalesced.
-__global__ void wt_from_s_kernel(double* WT, const double* S, const double* G, Dims d) {
+template <typename TS, typename TG>
+__global__ void wt_from_s_kernel(double* WT, const TS* S, const TG* G, Dims d) {
   extern __shared__ double tile[];  // [64][O+1]
   const int O = d.O, V = d.V;
   const long mb = blockIdx.x;
   const int m = (int)(mb / V), b = (int)(mb % V);
   const int e0 = blockIdx.y * 64;
   const int ne = min(64, V - e0);
-  const double* src = S + ((long)mb * V + e0) * O;
+  const TS* src = S + ((long)mb * V + e0) * O;
   for (int idx = threadIdx.x; idx < ne * O; idx += blockDim.x) {
     const int el = idx / O, j = idx % O;
-    tile[el * (O + 1) + j] = src[idx];
+    tile[el * (O + 1) + j] = (double)src[idx];
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < O * 64; idx += blockDim.x) {
     const int j = idx / 64, el = idx % 64;
     if (el >= ne) continue;
     const long off = ((long)j * V + b) * d.ov + (long)m * V + e0 + el;
-    WT[off] = tile[el * (O + 1) + j] + G[off];
+    WT[off] = tile[el * (O + 1) + j] + (double)G[off];
   }
 }
 
 // Fae_t1[a][e] = -sum_m S[(m,a,e)][m]
-__global__ void fae_t1_from_s_kernel(double* F, const double* S, Dims d) {
+template <typename TS>
+__global__ void fae_t1_from_s_kernel(double* F, const TS* S, Dims d) {
   const long n = (long)d.V * d.V;
   GRID_LOOP(idx, n) {
     long a = idx / d.V, e = idx % d.V;
     double s = 0.0;
-    for (int m = 0; m < d.O; ++m) s += S[(((long)m * d.V + a) * d.V + e) * d.O + m];
+    for (int m = 0; m < d.O; ++m) s += (double)S[(((long)m * d.V + a) * d.V + e) * d.O + m];
     F[idx] = -s;
   }
 }
@@ -329,29 +372,55 @@ __global__ void r1_init_kernel(double* R1, const double* f, const double* t1, Di
 }
 
 // ---------------------------------------------------------------- assembly / update
-__global__ void assemble_t2_kernel(double* out, double* err, const double* Rp, const double* X, const double* Y,
-                                   const double* Z, const double* f, const double* t2o, int mode, Dims d) {
-  const long n = d.po * d.pv;
-  const long N = d.O + d.V;
-  GRID_LOOP(idx, n) {
-    long ij = idx / d.pv, ab = idx % d.pv;
-    int i, j, a, b;
-    pair_decode(ij, i, j);
-    pair_decode(ab, a, b);
-    const long ia = (long)i * d.V + a, ib = (long)i * d.V + b, ja = (long)j * d.V + a, jb = (long)j * d.V + b;
-    double r = Rp[idx];
-    r += X[ia * d.ov + jb] - X[ja * d.ov + ib] - X[ib * d.ov + ja] + X[jb * d.ov + ia];
-    const double* y = Y + ij * d.V * d.V;
-    r += y[(long)a * d.V + b] - y[(long)b * d.V + a];
+// T2 assembly.  With Q(a,b) = X[(i,a),(j,b)] - X[(j,a),(i,b)] + Y[ij][a][b]
+// (rows of X and Y read contiguously along b), the P(ij)P(ab) ring and P(ab)
+// terms are r(a,b) = Q(a,b) - Q(b,a); the P(ij) term is Z[i][j][ab] - Z[j][i][ab].
+// Block (ij, tile pair ta <= tb of 32 x 32): Q tiles (ta,tb) and (tb,ta) are
+// staged in LDS and the packed outputs a < b of the tile written.
+// mode 0: out = Rp + r (residual); mode 1: out = (Rp + r) / D, err = out - t2o.
+// out may alias Rp (each element is read and written by the same thread).
+constexpr int kAsmT = 32;
+__global__ __launch_bounds__(256) void assemble_t2_kernel(double* out, double* err, const double* Rp,
+                                                          const double* __restrict__ X, const double* __restrict__ Y,
+                                                          const double* __restrict__ Z, const double* f,
+                                                          const double* t2o, int mode, Dims d) {
+  __shared__ double Qab[kAsmT][kAsmT + 1], Qba[kAsmT][kAsmT + 1];
+  const long V = d.V, ov = d.ov, N = d.O + d.V;
+  const long ij = blockIdx.x;
+  const int p = blockIdx.y;
+  int tb = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+  while ((tb + 1) * (tb + 2) / 2 <= p) ++tb;
+  while (tb * (tb + 1) / 2 > p) --tb;
+  const int ta = p - tb * (tb + 1) / 2;
+  const int a0 = ta * kAsmT, b0 = tb * kAsmT;
+  int i, j;
+  pair_decode(ij, i, j);
+  const double* Yij = Y + ij * V * V;
+  for (int idx = threadIdx.x; idx < 2 * kAsmT * kAsmT; idx += blockDim.x) {
+    const int which = idx / (kAsmT * kAsmT), r = (idx / kAsmT) % kAsmT, c = idx % kAsmT;
+    const long row = (which ? b0 : a0) + r, col = (which ? a0 : b0) + c;
+    double q = 0.0;
+    if (row < V && col < V)
+      q = X[(i * V + row) * ov + j * V + col] - X[(j * V + row) * ov + i * V + col] + Yij[row * V + col];
+    if (which) Qba[r][c] = q;
+    else Qab[r][c] = q;
+  }
+  __syncthreads();
+  const double fi = f[(long)i * N + i], fj = f[(long)j * N + j];
+  for (int idx = threadIdx.x; idx < kAsmT * kAsmT; idx += blockDim.x) {
+    const int lb = idx / kAsmT, la = idx % kAsmT;
+    const long a = a0 + la, b = b0 + lb;
+    if (a >= b || b >= V) continue;
+    const long ab = pidx(a, b), e = ij * d.pv + ab;
+    double r = Rp[e] + (Qab[la][lb] - Qba[lb][la]);
     r += Z[((long)i * d.O + j) * d.pv + ab] - Z[((long)j * d.O + i) * d.pv + ab];
     if (mode == 0) {
-      out[idx] = r;
+      out[e] = r;
     } else {
-      double D = f[(long)i * N + i] + f[(long)j * N + j] - f[(long)(d.O + a) * N + d.O + a] -
-                 f[(long)(d.O + b) * N + d.O + b];
-      double t = r / D;
-      out[idx] = t;
-      err[idx] = t - t2o[idx];
+      const double D = fi + fj - f[(d.O + a) * N + d.O + a] - f[(d.O + b) * N + d.O + b];
+      const double t = r / D;
+      out[e] = t;
+      err[e] = t - t2o[e];
     }
   }
 }
@@ -553,14 +622,26 @@ void k_build_tau(double* tau, const double* t2, const double* t1, Dims d, hipStr
   LAUNCH(build_tau_kernel, d.po * d.pv, tau, t2, t1, d);
 }
 void k_unpack_occ(double* T2u, double* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
-  LAUNCH(unpack_occ_kernel, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
+  LAUNCH(unpack_occ_kernel<double>, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
 }
-void k_unpack_vir(double* Tv, const double* t2, Dims d, hipStream_t s) {
-  LAUNCH(unpack_vir_kernel, d.po * d.V * d.V, Tv, t2, d);
+void k_unpack_occ_f32(float* T2u, float* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
+  LAUNCH(unpack_occ_kernel<float>, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
 }
-void k_unpack_ring(double* Tr, const double* t2, Dims d, hipStream_t s) {
-  LAUNCH(unpack_ring_kernel, d.ov * d.ov, Tr, t2, d);
+void k_unpack_vir_ring(double* Tv, double* Tr, const double* t2, Dims d, hipStream_t s) {
+  RingOut r;
+  r.d = Tr; r.ld = d.ov;
+  dim3 grid((unsigned)(d.po + (Tr ? d.O : 0)), (unsigned)((d.V + kUnpackA - 1) / kUnpackA));
+  hipLaunchKernelGGL(unpack_pairs_kernel<double>, grid, dim3(kThreads), kUnpackA * (d.V + 1) * sizeof(double), s,
+                     Tv, r, t2, d);
 }
+void k_unpack_vir_ring_lp(float* Tv, const RingOut& tr, const double* t2, Dims d, hipStream_t s) {
+  const bool ring = tr.d || tr.f || tr.h;
+  dim3 grid((unsigned)(d.po + (ring ? d.O : 0)), (unsigned)((d.V + kUnpackA - 1) / kUnpackA));
+  hipLaunchKernelGGL(unpack_pairs_kernel<float>, grid, dim3(kThreads), kUnpackA * (d.V + 1) * sizeof(double), s,
+                     Tv, tr, t2, d);
+}
+void k_unpack_vir(double* Tv, const double* t2, Dims d, hipStream_t s) { k_unpack_vir_ring(Tv, nullptr, t2, d, s); }
+void k_unpack_ring(double* Tr, const double* t2, Dims d, hipStream_t s) { k_unpack_vir_ring(nullptr, Tr, t2, d, s); }
 
 void k_fme_init(double* Fme, const double* f, Dims d, hipStream_t s) { LAUNCH(fme_init_kernel, d.ov, Fme, f, d); }
 void k_w1_finish(double* W1, const double* Q, Dims d, hipStream_t s) { LAUNCH(w1_finish_kernel, d.po * d.po, W1, Q, d); }
@@ -573,10 +654,19 @@ void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s) {
 void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s) { LAUNCH(w2_finish_kernel, d.po * d.ov, W2, U, d); }
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s) {
   dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
-  hipLaunchKernelGGL(wt_from_s_kernel, grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s, WT, S, G, d);
+  hipLaunchKernelGGL((wt_from_s_kernel<double, double>), grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s,
+                     WT, S, G, d);
+}
+void k_wt_from_s_f32(double* WT, const float* S, const float* G, Dims d, hipStream_t s) {
+  dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
+  hipLaunchKernelGGL((wt_from_s_kernel<float, float>), grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s,
+                     WT, S, G, d);
 }
 void k_fae_t1_from_s(double* F, const double* S, Dims d, hipStream_t s) {
-  LAUNCH(fae_t1_from_s_kernel, (long)d.V * d.V, F, S, d);
+  LAUNCH(fae_t1_from_s_kernel<double>, (long)d.V * d.V, F, S, d);
+}
+void k_fae_t1_from_s_f32(double* F, const float* S, Dims d, hipStream_t s) {
+  LAUNCH(fae_t1_from_s_kernel<float>, (long)d.V * d.V, F, S, d);
 }
 void k_fae_t1_extract(double* F, const double* WT, Dims d, hipStream_t s) {
   LAUNCH(fae_t1_kernel, (long)d.V * d.V, F, WT, d);
@@ -597,7 +687,9 @@ void k_r1_init(double* R1, const double* f, const double* t1, Dims d, hipStream_
 
 void k_assemble_t2(double* out, double* err, const double* Rp, const double* X, const double* Y, const double* Z,
                    const double* f, const double* t2_old, int mode, Dims d, hipStream_t s) {
-  LAUNCH(assemble_t2_kernel, d.po * d.pv, out, err, Rp, X, Y, Z, f, t2_old, mode, d);
+  const long nt = (d.V + kAsmT - 1) / kAsmT;
+  dim3 grid((unsigned)d.po, (unsigned)(nt * (nt + 1) / 2));
+  hipLaunchKernelGGL(assemble_t2_kernel, grid, dim3(kThreads), 0, s, out, err, Rp, X, Y, Z, f, t2_old, mode, d);
 }
 void k_mp2_guess(double* amp, const double* oovv, const double* f, Dims d, hipStream_t s) {
   LAUNCH(mp2_guess_kernel, d.ov + d.po * d.pv, amp, oovv, f, d);

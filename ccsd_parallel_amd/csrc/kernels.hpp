@@ -46,6 +46,19 @@ void k_build_tau(double* tau, const double* t2, const double* t1, Dims d, hipStr
 void k_unpack_occ(double* T2u, double* tauu, const double* t2, const double* tau, Dims d, hipStream_t s);
 void k_unpack_vir(double* Tv, const double* t2, Dims d, hipStream_t s);
 void k_unpack_ring(double* Tr, const double* t2, Dims d, hipStream_t s);
+// Tv and Tring (ring layout, with its zero diagonal blocks) in one pass; either may be null.
+void k_unpack_vir_ring(double* Tv, double* Tr, const double* t2, Dims d, hipStream_t s);
+// Destination of the ring layout: FP64, FP32 or a BF16 hi/lo pair, rows of
+// length ld (>= ov; padding columns are left untouched).
+struct RingOut {
+  double* d = nullptr;
+  float* f = nullptr;
+  uint16_t *h = nullptr, *l = nullptr;
+  long ld = 0;
+};
+// Low-precision iterations: FP32 occupied / virtual unpacks, ring layout as given
+void k_unpack_occ_f32(float* T2u, float* tauu, const double* t2, const double* tau, Dims d, hipStream_t s);
+void k_unpack_vir_ring_lp(float* Tv, const RingOut& tr, const double* t2, Dims d, hipStream_t s);
 
 // -- intermediates ------------------------------------------------------------------
 void k_fme_init(double* Fme, const double* f, Dims d, hipStream_t s);
@@ -56,7 +69,9 @@ void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s);
 void k_fae_t1_extract(double* Fae_t1, const double* WT, Dims d, hipStream_t s);
 // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f  ->  WT[(j,b),(m,e)] = S + <mb||ej>;  F_ae t1 part from S
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s);
+void k_wt_from_s_f32(double* WT, const float* S, const float* G, Dims d, hipStream_t s);
 void k_fae_t1_from_s(double* Fae_t1, const double* S, Dims d, hipStream_t s);
+void k_fae_t1_from_s_f32(double* F, const float* S, Dims d, hipStream_t s);
 void k_fmi_init(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d, hipStream_t s);
 void k_fmi_finish(double* Fmi, const double* Fme, const double* t1, Dims d, hipStream_t s);
 void k_fae_init(double* Fae, const double* f, const double* Fae_t1, const double* Fme, const double* t1, Dims d,

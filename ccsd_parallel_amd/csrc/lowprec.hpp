@@ -1,23 +1,59 @@
-// Mixed-precision operands for the four dominant contractions
-// (pp ladder, ring, W_mbej quadratic term, W_mbij tau term).
+// Mixed-precision operands and contractions of the low-precision iterations.
 //
-// Operands are stored k-contiguous with a leading dimension padded to a
-// multiple of 64 (zero filled), so the low-precision GEMMs run without K
-// bounds checks and with 16-byte vector loads.  Results are produced in FP32
-// and added into the FP64 destination.
+// The four dominant contractions (pp ladder, ring, W_mbej quadratic term,
+// W_mbij tau term) run as FP32 MFMA GEMMs or BF16 hi/lo x3 GEMMs on operands
+// stored k-contiguous with a leading dimension padded to a multiple of 64
+// (zero filled), so those kernels need no K bounds checks.  Every other
+// contraction of the residual runs on FP32 copies of its operands (same
+// layouts as the FP64 arrays) through the generic MFMA engine.  All products
+// accumulate in FP32 and land in the FP64 intermediates directly (fused
+// epilogues); energy, denominators, DIIS and the convergence test stay FP64.
 #pragma once
+#include <vector>
+
 #include "engine.hpp"
 
 namespace ccsd {
 
+struct LowPrec {
+  int have_fp32 = 0, have_bf16 = 0, have_sec = 0;
+  long ld_pv = 0, ld_ov = 0;  // padded leading dims for K = pv and K = ov
+  // FP32 operands of the dominant contractions (k-contiguous, padded); ovvv32
+  // and tau32 also serve the secondary contractions in both modes
+  float *vvvv32 = nullptr, *ovvv32 = nullptr, *Doovv32 = nullptr;
+  float *tau32 = nullptr, *Tring32 = nullptr, *WT32 = nullptr;
+  // BF16 hi/lo operands
+  uint16_t *vvvv_h = nullptr, *vvvv_l = nullptr, *ovvv_h = nullptr, *ovvv_l = nullptr;
+  uint16_t *Doovv_h = nullptr, *Doovv_l = nullptr;
+  uint16_t *tau_h = nullptr, *tau_l = nullptr, *Tring_h = nullptr, *Tring_l = nullptr, *WT_h = nullptr,
+           *WT_l = nullptr;
+  // FP32 copies for the secondary contractions, same layouts as the FP64 arrays:
+  // static integrals ...
+  float *oovv_f = nullptr, *oovv_vu_f = nullptr, *oovv_ou_f = nullptr, *G_f = nullptr, *ovvv_u_f = nullptr;
+  // ... and per-iteration amplitudes / intermediates
+  float *t1_f = nullptr, *T2u_f = nullptr, *tauu_f = nullptr, *Tv_f = nullptr;
+  float *W1_f = nullptr, *W3_f = nullptr, *W3u_f = nullptr, *W2_f = nullptr, *Fae_f = nullptr, *Fmi_f = nullptr;
+  std::vector<void*> allocs;
+};
+
 size_t lowprec_plan_bytes(const Dims& d, int precision);
 int lowprec_allocate(Ctx* c, int precision);
 void lowprec_free(Ctx* c);
+// Low-precision amplitude operands (tau, Tring, t1) of the current iteration.
 int lowprec_amplitudes(Ctx* c, int precision);
+// Destination of the ring-layout unpack (Tring as the mode's operand, padded rows).
+RingOut lowprec_ring_out(Ctx* c, int precision);
+// F_me += <mn||ef> t_n^f and R1 += t_im^ae F_me on the low-precision copies.
+int lowprec_fme(Ctx* c, int precision, const double* t1);
+int lowprec_r1_ring(Ctx* c, int precision);
 int lowprec_w2(Ctx* c, int precision);
 int lowprec_wmbej(Ctx* c, int precision);
 int lowprec_ladder(Ctx* c, int precision);
 int lowprec_ring(Ctx* c, int precision);
+
+// FP32 GEMM with the operands A32 / B32 (strides of g unless overridden by the
+// caller in a copy of g) writing g's FP64 destination.
+GemmDesc<float> f32_desc(const GemmDesc<double>& g, const float* A32, const float* B32);
 
 // Self-test entry used by ccsd_selftest_gemm: C(MxN) (FP64) = alpha * A B + beta C
 // with A given k-contiguous [M][K] and B given k-contiguous [N][K] (device FP64).

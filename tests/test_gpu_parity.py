@@ -207,6 +207,33 @@ def test_low_precision_fixed_point_is_near_fp64(prec, floor):
     assert abs(tr[-1] - e64) <= floor * max(1.0, abs(e64))
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 5e-4)])
+def test_low_precision_update_matches_fp64_update(prec, tol):
+    """One amplitude update from the same amplitudes (T1 nonzero everywhere) in
+    FP64 and in low precision.  Every contraction of the low-precision residual
+    -- the BF16/FP32 dominant GEMMs and the FP32 secondary ones -- enters the
+    new amplitudes, which must agree to the rounding level of the mode."""
+    capi = _ctx()
+    O, V, seed = 8, 36, 77
+    sc = norm_scale(O, V)
+    rng = np.random.default_rng(5)
+    with capi.Context.synthetic(O, V, seed, sc) as ctx:
+        ctx.iterate(2)
+        t1, t2 = ctx.get_amplitudes()
+        t1 = t1 + 0.01 * rng.standard_normal(t1.shape)
+        out = {}
+        for p in ("fp64", prec, "fp64"):
+            ctx.set_amplitudes(t1, t2)
+            ctx.iterate(1, p)
+            out.setdefault(p, []).append(ctx.get_amplitudes())
+    (a1, a2), (a1b, a2b) = out["fp64"]
+    b1, b2 = out[prec][0]
+    assert np.array_equal(a2, a2b) and np.array_equal(a1, a1b)  # FP64 after low precision is unaffected
+    e2 = np.max(np.abs(b2 - a2)) / np.max(np.abs(a2))
+    e1 = np.max(np.abs(b1 - a1)) / np.max(np.abs(a1))
+    assert 0 < e2 <= tol and e1 <= tol, (e1, e2)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_mixed_precision_converges_to_fp64(prec):
     capi = _ctx()
