@@ -45,24 +45,24 @@ def flops_per_iteration(O, V):
     )
 
 
-def cpu_baseline(O, V, seed, scale, budget_s=15.0):
+def cpu_baseline(O, V, seed, scale, budget_s=12.0):
     """CPU oracle on a bounded sample of the workload.
 
     The sample is the pp-ladder contraction of the FP64 oracle (numpy BLAS,
-    all host threads) over a 64-column slice of <ef||ab>. The slice is made by
-    the same counter-based generator. The rate is extrapolated to one full
-    iteration by flop count."""
+    all host threads) over a 256-column slice of <ef||ab>, repeated for about
+    budget_s seconds. The slice is made by the same counter-based generator.
+    The rate is extrapolated to one full iteration by flop count."""
     from oracle import synth
     po, pv = p2(O), p2(V)
     rng = np.random.default_rng(0)
     tau = rng.standard_normal((po, pv)) * 1e-3
-    ncols = 64
+    ncols = 256
     t0 = time.perf_counter()
     blk = synth.packed_block("vvvv", O, V, seed, scale, rows=(0, ncols)).T  # symmetric block: rows == columns
     gen_s = time.perf_counter() - t0
     reps, el = 0, 0.0
     t0 = time.perf_counter()
-    while el < budget_s and reps < 50:
+    while el < budget_s and reps < 100000:
         _ = tau @ blk
         reps += 1
         el = time.perf_counter() - t0

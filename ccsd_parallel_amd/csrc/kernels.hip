@@ -154,16 +154,16 @@ __global__ void unpack_occ_kernel(T* T2u, T* tauu, const double* t2, const doubl
 // Blocks with k1 >= P2(O) write the zero diagonal blocks Tr[(i,a),(i,e)].
 constexpr int kUnpackA = 16;
 __device__ __forceinline__ void ring_store(const RingOut& o, long row, long col, double v) {
-  const long off = row * o.ld + col;
   if (o.d) {
-    o.d[off] = v;
+    o.d[row * o.ld + col] = v;
   } else if (o.f) {
-    o.f[off] = (float)v;
+    o.f[row * o.ld + col] = (float)v;
   } else {
     uint16_t h, l;
     bf16_split(v, h, l);
-    o.h[off] = h;
-    o.l[off] = l;
+    const long i = row * 2 * o.ld + (col >> 3) * 16 + (col & 7);
+    o.b[i] = h;
+    o.b[i + 8] = l;
   }
 }
 
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void unpack_pairs_kernel(TV* Tv, RingOut Tr, c
   extern __shared__ double tile[];  // kUnpackA x (V + 1)
   const int V = (int)d.V, LD = V + 1;
   const long k1 = blockIdx.x;
-  const bool ring = Tr.d || Tr.f || Tr.h;
+  const bool ring = Tr.d || Tr.f || Tr.b;
   const int a0 = blockIdx.y * kUnpackA;
   const int na = min(kUnpackA, V - a0);
   if (k1 >= d.po) {
@@ -635,7 +635,7 @@ void k_unpack_vir_ring(double* Tv, double* Tr, const double* t2, Dims d, hipStre
                      Tv, r, t2, d);
 }
 void k_unpack_vir_ring_lp(float* Tv, const RingOut& tr, const double* t2, Dims d, hipStream_t s) {
-  const bool ring = tr.d || tr.f || tr.h;
+  const bool ring = tr.d || tr.f || tr.b;
   dim3 grid((unsigned)(d.po + (ring ? d.O : 0)), (unsigned)((d.V + kUnpackA - 1) / kUnpackA));
   hipLaunchKernelGGL(unpack_pairs_kernel<float>, grid, dim3(kThreads), kUnpackA * (d.V + 1) * sizeof(double), s,
                      Tv, tr, t2, d);

@@ -48,12 +48,13 @@ void k_unpack_vir(double* Tv, const double* t2, Dims d, hipStream_t s);
 void k_unpack_ring(double* Tr, const double* t2, Dims d, hipStream_t s);
 // Tv and Tring (ring layout, with its zero diagonal blocks) in one pass; either may be null.
 void k_unpack_vir_ring(double* Tv, double* Tr, const double* t2, Dims d, hipStream_t s);
-// Destination of the ring layout: FP64, FP32 or a BF16 hi/lo pair, rows of
-// length ld (>= ov; padding columns are left untouched).
+// Destination of the ring layout: FP64 or FP32 rows of length ld (>= ov), or
+// BF16 hi/lo rows interleaved per 8-element chunk (row r at r * 2 * ld);
+// padding columns are left untouched.
 struct RingOut {
   double* d = nullptr;
   float* f = nullptr;
-  uint16_t *h = nullptr, *l = nullptr;
+  uint16_t* b = nullptr;
   long ld = 0;
 };
 // Low-precision iterations: FP32 occupied / virtual unpacks, ring layout as given

@@ -35,22 +35,42 @@ __global__ void conv_f32_kernel(float* dst, const double* src, long rows, long K
   }
 }
 
-__global__ void conv_bf16x2_kernel(uint16_t* hi, uint16_t* lo, const double* src, long rows, long K, long sld,
-                                   long ldk, double scale) {
-  const long n = rows * ldk;
+// BF16 hi/lo operands are stored interleaved per 8-element chunk: row r of a
+// matrix with leading dimension ld (multiple of 64) holds, for chunk c = k/8,
+// the 8 hi values followed by the 8 lo values at r*2*ld + 16*c.  One k-tile of
+// 32 elements of a row is then one contiguous 128-byte line.
+__device__ __forceinline__ long il_index(long row, long k, long ld) { return row * 2 * ld + (k >> 3) * 16 + (k & 7); }
+
+// dst (interleaved, rows x ld) = split(scale * src[r*sld + k]) for k < K, 0 beyond;
+// one thread per 8-element chunk (two 16-byte stores).
+__global__ void conv_bf16il_kernel(uint16_t* dst, const double* src, long rows, long K, long sld, long ld,
+                                   double scale) {
+  const long chunks = ld / 8, n = rows * chunks;
   GRID_LOOP(idx, n) {
-    long r = idx / ldk, k = idx % ldk;
-    uint16_t h = 0, l = 0;
-    if (k < K) bf16_split(scale * src[r * sld + k], h, l);
-    hi[idx] = h;
-    lo[idx] = l;
+    const long r = idx / chunks, c = idx % chunks;
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long k = c * 8 + e;
+      h[e] = 0;
+      l[e] = 0;
+      if (k < K) bf16_split(scale * src[r * sld + k], h[e], l[e]);
+    }
+    uint4 hv, lv;
+    hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
+    hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
+    lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
+    lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
+    uint4* o = (uint4*)(dst + r * 2 * ld + c * 16);
+    o[0] = hv;
+    o[1] = lv;
   }
 }
 
-// y[m] = alpha * sum_k A[m][k] x[k] + beta * y[m], A given as FP32 (A32) or as a
-// BF16 hi/lo pair (row stride ld, a multiple of 64); one wavefront per row,
+// y[m] = alpha * sum_k A[m][k] x[k] + beta * y[m], A given as FP32 (A32, row
+// stride ld) or as interleaved BF16 hi/lo rows (Ab); one wavefront per row,
 // 8 consecutive k per lane (16-byte loads) and a scalar tail.
-__global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ah, const uint16_t* Al, long ld,
+__global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ab, long ld,
                                const double* __restrict__ x, long M, long K, double alpha, double beta) {
   const int lane = threadIdx.x & 63;
   const long row = blockIdx.x * (long)(blockDim.x / 64) + (threadIdx.x >> 6);
@@ -66,10 +86,9 @@ __global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ah, 
     }
     for (long k = K8 + lane; k < K; k += 64) s += (double)a[k] * x[k];
   } else {
-    const uint16_t* h = Ah + row * ld;
-    const uint16_t* l = Al + row * ld;
+    const uint16_t* a = Ab + row * 2 * ld;
     for (long k0 = lane * 8; k0 < K8; k0 += 512) {
-      const uint4 hv = *(const uint4*)(h + k0), lv = *(const uint4*)(l + k0);
+      const uint4 hv = *(const uint4*)(a + 2 * k0), lv = *(const uint4*)(a + 2 * k0 + 8);
       const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w}, lw[4] = {lv.x, lv.y, lv.z, lv.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -78,7 +97,10 @@ __global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ah, 
         s += a0 * x[k0 + 2 * q] + a1 * x[k0 + 2 * q + 1];
       }
     }
-    for (long k = K8 + lane; k < K; k += 64) s += ((double)bf16_to_f(h[k]) + (double)bf16_to_f(l[k])) * x[k];
+    for (long k = K8 + lane; k < K; k += 64) {
+      const long i = il_index(0, k, ld);
+      s += ((double)bf16_to_f(a[i]) + (double)bf16_to_f(a[i + 8])) * x[k];
+    }
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) y[row] = alpha * s + (beta != 0.0 ? beta * y[row] : 0.0);
@@ -86,21 +108,22 @@ __global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ah, 
 
 // Output of a BF16 hi/lo product: FP32 C[row*ldc + col] = acc, or (C64 set)
 // the FP64 C64[row*ldc + col] = alpha * acc + beta * C64, optionally also
-// split into the hi/lo pair Oh/Ol[row*ldo + col] for use as a later operand.
+// written as interleaved hi/lo rows Ob (leading dimension ldo) for use as a
+// later operand.
 struct Bf16Out {
   float* C = nullptr;
   double* C64 = nullptr;
   long ldc = 0;
   double alpha = 1.0, beta = 0.0;
-  uint16_t *Oh = nullptr, *Ol = nullptr;
+  uint16_t* Ob = nullptr;
   long ldo = 0;
 };
 
 struct Bf16Args {
-  const uint16_t *Ahi, *Alo, *Bhi, *Blo;
+  const uint16_t *A, *B;  // interleaved hi/lo rows, leading dimensions lda / ldb (elements)
   long lda, ldb;
-  int M, N, Kp;  // Kp: K rounded up to 32 (operands zero padded)
-  int tiles_m, tiles_n, tiles_pad;
+  int M, N, Kp;           // Kp: K rounded up to 32 (operands zero padded)
+  int tiles_m, tiles_n, tiles_pad, group;
   Bf16Out out;
 };
 
@@ -110,70 +133,77 @@ __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, f
     double v = o.alpha * (double)acc;
     if (o.beta != 0.0) v += o.beta * (*c);
     *c = v;
-    if (o.Oh) {
+    if (o.Ob) {
       uint16_t h, l;
       bf16_split(v, h, l);
-      o.Oh[(long)row * o.ldo + col] = h;
-      o.Ol[(long)row * o.ldo + col] = l;
+      const long i = il_index(row, col, o.ldo);
+      o.Ob[i] = h;
+      o.Ob[i + 8] = l;
     }
   } else {
     o.C[(long)row * o.ldc + col] = acc;
   }
 }
 
-// C[m][n] = sum_k A[m][k] B[n][k] with A = Ahi + Alo, B = Bhi + Blo, dropping lo*lo.
+// C[m][n] = sum_k A[m][k] B[n][k] with A = Ahi + Alo, B = Bhi + Blo, dropping
+// lo*lo: three v_mfma_f32_16x16x32_bf16 per 16x16x32 step, FP32 accumulation.
+//
+// Block tile BM x BN over WM x WN waves (wave tile (BM/WM) x (BN/WN)), k-tiles
+// of 32.  Each thread moves 16-byte pieces of the interleaved rows (8 per row
+// per k-tile: 4 hi + 4 lo) through registers into double-buffered LDS, laid out
+// [hi|lo][row][k] with an 80-byte row pitch (16 lanes reading one k-slice hit
+// distinct banks).  The next k-tile's loads are issued unconditionally before
+// the current tile's MFMAs (the last iteration reloads its own tile into the
+// idle buffer) so they stay in flight across the compute; rows past M / N are
+// clamped to the last row (they only feed accumulator rows that are never
+// stored).  At 256 x 256 the LDS is 160 KB and a block holds one CU.
+//
+// Tile order: XCD-aware (linear workgroup x runs on XCD x % 8, so XCD j takes a
+// contiguous range of tiles); with group > 0 the tiles of that range are
+// walked in groups of `group` tile rows so the 32 concurrent tiles of an XCD
+// share operand panels in its L2.
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
   constexpr int NT = 64 * WM * WN;
   constexpr int BK = 32;
-  constexpr int LDK = BK + 8;  // 80-byte row pitch: 16 lanes of one k-slice hit distinct banks
+  constexpr int LDK = BK + 8;
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
-  constexpr int CA = 2 * BM * 4 / NT;  // 16-byte chunks per thread (hi+lo)
-  constexpr int CB = 2 * BN * 4 / NT;
-  static_assert(CA * NT == 2 * BM * 4 && CB * NT == 2 * BN * 4, "tile/threads mismatch");
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * LDK];
+  constexpr int CA = BM * 8 / NT, CB = BN * 8 / NT;  // 16-byte pieces per thread per k-tile
+  static_assert(CA * NT == BM * 8 && CB * NT == BN * 8, "tile/threads mismatch");
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDK];
 
-  const int gx = p.tiles_pad;  // XCD-aware order when a multiple of 8 (see bf16x3_nt)
+  const int gx = p.tiles_pad;
   const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
   if (tile >= p.tiles_m * p.tiles_n) return;
-  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  if (p.group > 0) {
+    const int per = p.group * p.tiles_n, g = tile / per, first = g * p.group;
+    const int gs = min(p.tiles_m - first, p.group), r = tile % per;
+    tm = first + r % gs;
+    tn = r / gs;
+  }
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
 
+  // piece u = it*NT + tid: row u/8, piece q = u%8 -> chunk q/2, hi (q even) or lo
+  const uint16_t* pa[CA];
+  const uint16_t* pb[CB];
+  int sa[CA], sb[CB];
+#pragma unroll
+  for (int it = 0; it < CA; ++it) {
+    const int u = it * NT + tid, m = u >> 3, q = u & 7;
+    pa[it] = p.A + (long)min(m0 + m, p.M - 1) * 2 * p.lda + q * 8;
+    sa[it] = (q & 1) * (BM * LDK) + m * LDK + (q >> 1) * 8;
+  }
+#pragma unroll
+  for (int it = 0; it < CB; ++it) {
+    const int u = it * NT + tid, n = u >> 3, q = u & 7;
+    pb[it] = p.B + (long)min(n0 + n, p.N - 1) * 2 * p.ldb + q * 8;
+    sb[it] = (q & 1) * (BN * LDK) + n * LDK + (q >> 1) * 8;
+  }
   uint4 ra[CA], rb[CB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int it = 0; it < CA; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
-      const uint16_t* src = (hl ? p.Alo : p.Ahi) + (long)(m0 + m) * p.lda + k0 + ch * 8;
-      ra[it] = (m0 + m < p.M) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int it = 0; it < CB; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
-      const uint16_t* src = (hl ? p.Blo : p.Bhi) + (long)(n0 + n) * p.ldb + k0 + ch * 8;
-      rb[it] = (n0 + n < p.N) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int it = 0; it < CA; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
-      *(uint4*)&sA[hl][m * LDK + ch * 8] = ra[it];
-    }
-#pragma unroll
-    for (int it = 0; it < CB; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
-      *(uint4*)&sB[hl][n * LDK + ch * 8] = rb[it];
-    }
-  };
-
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -181,114 +211,23 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
   const int nk = p.Kp / BK;
-  if (nk > 0) load(0);
+#pragma unroll
+  for (int it = 0; it < CA; ++it) ra[it] = *(const uint4*)(pa[it]);
+#pragma unroll
+  for (int it = 0; it < CB; ++it) rb[it] = *(const uint4*)(pb[it]);
+#pragma unroll
+  for (int it = 0; it < CA; ++it) *(uint4*)&sA[0][0][sa[it]] = ra[it];
+#pragma unroll
+  for (int it = 0; it < CB; ++it) *(uint4*)&sB[0][0][sb[it]] = rb[it];
+  __syncthreads();
+  const int kofs = 8 * (lane >> 4);
   for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();  // previous compute finished reading LDS
-    store();
-    __syncthreads();
-    if (kt + 1 < nk) load((kt + 1) * BK);
-    const int kofs = 8 * (lane >> 4);
-    bf16x8_t ah[TM], al[TM], bh[TN], bl[TN];
+    const long kn = 2L * min(kt + 1, nk - 1) * BK;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int r = (wm0 + i * 16 + (lane & 15)) * LDK + kofs;
-      ah[i] = *(const bf16x8_t*)&sA[0][r];
-      al[i] = *(const bf16x8_t*)&sA[1][r];
-    }
+    for (int it = 0; it < CA; ++it) ra[it] = *(const uint4*)(pa[it] + kn);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int r = (wn0 + j * 16 + (lane & 15)) * LDK + kofs;
-      bh[j] = *(const bf16x8_t*)&sB[0][r];
-      bl[j] = *(const bf16x8_t*)&sB[1][r];
-    }
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-      }
-  }
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-        int col = n0 + wn0 + j * 16 + (lane & 15);
-        if (row < p.M && col < p.N) bf16_store(p.out, row, col, acc[i][j][r]);
-      }
-}
-
-// Large-tile variant: 8 waves, each a 64 x 128 sub-tile (4 x 8 MFMA tiles, 128
-// accumulator registers), so one 256 x 256 (or 128 x 512) block tile reaches
-// ~190 flop per byte loaded -- the 128 x 128 tile above needs ~25 TB/s of L2
-// bandwidth to feed 2.5 PF/s.  Per k-step the A fragments of the wave are kept
-// in registers and the B fragments are streamed one 16-column tile at a time.
-// DBUF: double-buffered LDS (one barrier per k-tile; 160 KB at 256 x 256).
-template <int BM, int BN, int WM, int WN, bool DBUF>
-__global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_big_kernel(Bf16Args p) {
-  constexpr int NT = 64 * WM * WN;
-  constexpr int BK = 32;
-  constexpr int LDK = BK + 8;
-  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
-  constexpr int CA = 2 * BM * 4 / NT;
-  constexpr int CB = 2 * BN * 4 / NT;
-  static_assert(CA * NT == 2 * BM * 4 && CB * NT == 2 * BN * 4, "tile/threads mismatch");
-  constexpr int NBUF = DBUF ? 2 : 1;
-  __shared__ __attribute__((aligned(16))) uint16_t sA[NBUF][2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[NBUF][2][BN * LDK];
-
-  const int gx = p.tiles_pad;  // XCD-aware order when a multiple of 8 (see bf16x3_nt)
-  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
-  if (tile >= p.tiles_m * p.tiles_n) return;
-  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
-
-  uint4 ra[CA], rb[CB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int it = 0; it < CA; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
-      const uint16_t* src = (hl ? p.Alo : p.Ahi) + (long)(m0 + m) * p.lda + k0 + ch * 8;
-      ra[it] = (m0 + m < p.M) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int it = 0; it < CB; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
-      const uint16_t* src = (hl ? p.Blo : p.Bhi) + (long)(n0 + n) * p.ldb + k0 + ch * 8;
-      rb[it] = (n0 + n < p.N) ? *(const uint4*)src : make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int it = 0; it < CA; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BM * 4), rem = u % (BM * 4), m = rem >> 2, ch = rem & 3;
-      *(uint4*)&sA[buf][hl][m * LDK + ch * 8] = ra[it];
-    }
-#pragma unroll
-    for (int it = 0; it < CB; ++it) {
-      int u = it * NT + tid;
-      int hl = u / (BN * 4), rem = u % (BN * 4), n = rem >> 2, ch = rem & 3;
-      *(uint4*)&sB[buf][hl][n * LDK + ch * 8] = rb[it];
-    }
-  };
-
-  f32x4 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
-
-  auto compute = [&](int buf) {
-    const int kofs = 8 * (lane >> 4);
+    for (int it = 0; it < CB; ++it) rb[it] = *(const uint4*)(pb[it] + kn);
+    const int buf = kt & 1;
     bf16x8_t ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -308,29 +247,12 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_big_kernel(Bf16Args p)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
       }
     }
-  };
-
-  const int nk = p.Kp / BK;
-  if (nk > 0) {
-    load(0);
-    store(0);
+    const int nb = buf ^ 1;
+#pragma unroll
+    for (int it = 0; it < CA; ++it) *(uint4*)&sA[nb][0][sa[it]] = ra[it];
+#pragma unroll
+    for (int it = 0; it < CB; ++it) *(uint4*)&sB[nb][0][sb[it]] = rb[it];
     __syncthreads();
-  }
-  for (int kt = 0; kt < nk; ++kt) {
-    const bool more = kt + 1 < nk;
-    if (more) load((kt + 1) * BK);
-    if (DBUF) {
-      compute(kt & 1);
-      if (more) store((kt + 1) & 1);
-      __syncthreads();
-    } else {
-      compute(0);
-      __syncthreads();
-      if (more) {
-        store(0);
-        __syncthreads();
-      }
-    }
   }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -338,35 +260,41 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_big_kernel(Bf16Args p)
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-        int col = n0 + wn0 + j * 16 + (lane & 15);
+        const int row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn0 + j * 16 + (lane & 15);
         if (row < p.M && col < p.N) bf16_store(p.out, row, col, acc[i][j][r]);
       }
 }
 
-template <int BM, int BN, int WM, int WN, bool DBUF>
-hipError_t launch_big(Bf16Args p, hipStream_t s) {
+template <int BM, int BN, int WM, int WN>
+hipError_t launch_bf16(Bf16Args p, hipStream_t s) {
   p.tiles_m = (p.M + BM - 1) / BM;
   p.tiles_n = (p.N + BN - 1) / BN;
-  long tiles = (long)p.tiles_m * p.tiles_n;
+  const long tiles = (long)p.tiles_m * p.tiles_n;
   p.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
-  hipLaunchKernelGGL((bf16x3_nt_big_kernel<BM, BN, WM, WN, DBUF>), dim3(p.tiles_pad), dim3(64 * WM * WN), 0, s, p);
+  hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, WM, WN>), dim3(p.tiles_pad), dim3(64 * WM * WN), 0, s, p);
   return hipGetLastError();
 }
 
-// Tile choice by modelled time: padding x wave quantisation over 256 CUs,
-// divided by the tile's relative throughput (measured on MI355X).
-hipError_t bf16x3_nt(const uint16_t* Ahi, const uint16_t* Alo, long lda, const uint16_t* Bhi, const uint16_t* Blo,
-                     long ldb, const Bf16Out& out, int M, int N, int K, hipStream_t s, int force = -1) {
+// C = A B^T on interleaved BF16 hi/lo operands.  Tile shape by modelled time
+// (padding x wave quantisation over 256 CUs / relative throughput measured on
+// MI355X): 256 x 256 on 8 waves for large products, 128 x 128 on 4 waves for
+// small ones.  The grouped tile order pays on square products with many tile
+// rows (measured ~3% on 12000^3) and not on the P2(O)-row ones.
+// force (tests): 0 = 128 x 128, 1 = 256 x 256 automatic order, 2 = 256 x 256
+// plain order, 3 = 256 x 256 grouped order.
+hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, const Bf16Out& out, int M, int N,
+                     int K, hipStream_t s, int force = -1) {
   Bf16Args p;
-  p.Ahi = Ahi; p.Alo = Alo; p.Bhi = Bhi; p.Blo = Blo; p.out = out;
+  p.A = A; p.B = B; p.out = out;
   p.lda = lda; p.ldb = ldb;
   p.M = M; p.N = N; p.Kp = ((K + 31) / 32) * 32;
+  p.group = 0;
   struct Opt { int BM, BN; double rate; };
-  const Opt opts[3] = {{128, 128, 1.0}, {256, 256, 1.65}, {128, 512, 1.5}};
+  const Opt opts[2] = {{128, 128, 1.0}, {256, 256, 1.65}};
   int best = 0;
   double bt = 1e300;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < 2; ++c) {
     long tm = (M + opts[c].BM - 1) / opts[c].BM, tn = (N + opts[c].BN - 1) / opts[c].BN;
     double pad = double(tm * opts[c].BM) * double(tn * opts[c].BN) / (double(M) * N);
     double tiles = double(tm * tn);
@@ -374,21 +302,16 @@ hipError_t bf16x3_nt(const uint16_t* Ahi, const uint16_t* Alo, long lda, const u
     double t = pad * (rounds * 256.0 / tiles) / opts[c].rate;
     if (t < bt) { bt = t; best = c; }
   }
-  if (best == 1) best = 3;  // 256 x 256: the double-buffered variant measured ~1% faster
-  if (force >= 0 && force < 4) best = force;
-  switch (best) {
-    case 1: return launch_big<256, 256, 4, 2, false>(p, s);
-    case 2: return launch_big<128, 512, 2, 4, false>(p, s);
-    case 3: return launch_big<256, 256, 4, 2, true>(p, s);
-    default: break;
+  int group = (best == 1 && (M + 255) / 256 >= 16) ? 8 : 0;
+  if (force == 0) best = 0;
+  if (force >= 1 && force <= 3) {
+    best = 1;
+    if (force == 2) group = 0;
+    if (force == 3) group = 8;
   }
-  constexpr int BM = 128, BN = 128;
-  p.tiles_m = (M + BM - 1) / BM;
-  p.tiles_n = (N + BN - 1) / BN;
-  long tiles = (long)p.tiles_m * p.tiles_n;
-  p.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
-  hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, 2, 2>), dim3(p.tiles_pad), dim3(256), 0, s, p);
-  return hipGetLastError();
+  if (best == 0) return launch_bf16<128, 128, 2, 2>(p, s);
+  p.group = group;
+  return launch_bf16<256, 256, 4, 2>(p, s);
 }
 
 }  // namespace
@@ -434,6 +357,11 @@ static int lp_alloc(Ctx* c, T** p, long n) {
       return CCSD_ERR_HIP;                     \
     }                                          \
   } while (0)
+
+static void conv_il(Ctx* c, uint16_t* dst, const double* src, long rows, long K, long ld) {
+  hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(rows * ld / 8)), dim3(kThreads), 0, c->stream, dst, src, rows,
+                     K, K, ld, 1.0);
+}
 
 static void to_f32(Ctx* c, float* dst, const double* src, long n) {
   hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, dst, src, 1L, n, n, n, 1.0);
@@ -492,28 +420,18 @@ int lowprec_allocate(Ctx* c, int precision) {
     L->have_fp32 = 1;
   }
   if (precision == CCSD_PREC_BF16X3 && !L->have_bf16) {
-    TRY(lp_alloc(c, &L->vvvv_h, d.pv * L->ld_pv));
-    TRY(lp_alloc(c, &L->vvvv_l, d.pv * L->ld_pv));
-    TRY(lp_alloc(c, &L->ovvv_h, d.ov * L->ld_pv));
-    TRY(lp_alloc(c, &L->ovvv_l, d.ov * L->ld_pv));
-    TRY(lp_alloc(c, &L->Doovv_h, d.ov * L->ld_ov));
-    TRY(lp_alloc(c, &L->Doovv_l, d.ov * L->ld_ov));
-    TRY(lp_alloc(c, &L->tau_h, d.po * L->ld_pv));
-    TRY(lp_alloc(c, &L->tau_l, d.po * L->ld_pv));
-    TRY(lp_alloc(c, &L->Tring_h, d.ov * L->ld_ov));
-    TRY(lp_alloc(c, &L->Tring_l, d.ov * L->ld_ov));
-    TRY(lp_alloc(c, &L->WT_h, d.ov * L->ld_ov));
-    TRY(lp_alloc(c, &L->WT_l, d.ov * L->ld_ov));
-    // WT_h/WT_l (W_mbej epilogue) and Tring_h/Tring_l (unpack) are written for
-    // columns < ov only: zero the padding once
-    for (uint16_t* q : {L->WT_h, L->WT_l, L->Tring_h, L->Tring_l})
-      HIPTRY(hipMemsetAsync(q, 0, (size_t)d.ov * L->ld_ov * 2, s));
-    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.pv * L->ld_pv)), dim3(kThreads), 0, s, L->vvvv_h,
-                       L->vvvv_l, c->vvvv, d.pv, d.pv, d.pv, L->ld_pv, 1.0);
-    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.ov * L->ld_pv)), dim3(kThreads), 0, s, L->ovvv_h,
-                       L->ovvv_l, c->ovvv, d.ov, d.pv, d.pv, L->ld_pv, 1.0);
-    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, s, L->Doovv_h,
-                       L->Doovv_l, c->Doovv, d.ov, d.ov, d.ov, L->ld_ov, 1.0);
+    TRY(lp_alloc(c, &L->vvvv_b, 2 * d.pv * L->ld_pv));
+    TRY(lp_alloc(c, &L->ovvv_b, 2 * d.ov * L->ld_pv));
+    TRY(lp_alloc(c, &L->Doovv_b, 2 * d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->tau_b, 2 * d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->Tring_b, 2 * d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->WT_b, 2 * d.ov * L->ld_ov));
+    // WT_b (W_mbej epilogue) and Tring_b (unpack) are written for columns < ov
+    // only: zero the padding once
+    for (uint16_t* q : {L->WT_b, L->Tring_b}) HIPTRY(hipMemsetAsync(q, 0, (size_t)d.ov * L->ld_ov * 4, s));
+    conv_il(c, L->vvvv_b, c->vvvv, d.pv, d.pv, L->ld_pv);
+    conv_il(c, L->ovvv_b, c->ovvv, d.ov, d.pv, L->ld_pv);
+    conv_il(c, L->Doovv_b, c->Doovv, d.ov, d.ov, L->ld_ov);
     L->have_bf16 = 1;
   }
   hipError_t e = hipGetLastError();
@@ -531,16 +449,6 @@ void lowprec_free(Ctx* c) {
   c->lp = nullptr;
 }
 
-static void conv(Ctx* c, int prec, const double* src, long rows, long K, long ld, float* d32, uint16_t* dh,
-                 uint16_t* dl) {
-  long n = rows * ld;
-  if (prec == CCSD_PREC_FP32)
-    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, d32, src, rows, K, K, ld,
-                       1.0);
-  else
-    hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, dh, dl, src, rows, K,
-                       K, ld, 1.0);
-}
 
 int lowprec_amplitudes(Ctx* c, int prec) {
   TRY(lowprec_allocate(c, prec));
@@ -548,7 +456,7 @@ int lowprec_amplitudes(Ctx* c, int prec) {
   const Dims& d = c->d;
   hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.po * L->ld_pv)), dim3(kThreads), 0, c->stream, L->tau32, c->tau,
                      d.po, d.pv, d.pv, L->ld_pv, 1.0);
-  if (prec == CCSD_PREC_BF16X3) conv(c, prec, c->tau, d.po, d.pv, L->ld_pv, nullptr, L->tau_h, L->tau_l);
+  if (prec == CCSD_PREC_BF16X3) conv_il(c, L->tau_b, c->tau, d.po, d.pv, L->ld_pv);
   return CCSD_OK;
 }
 
@@ -559,39 +467,38 @@ RingOut lowprec_ring_out(Ctx* c, int prec) {
   if (prec == CCSD_PREC_FP32) {
     r.f = L->Tring32;
   } else {
-    r.h = L->Tring_h;
-    r.l = L->Tring_l;
+    r.b = L->Tring_b;
   }
   return r;
 }
 
-static int gemv_lp(Ctx* c, int prec, const float* A32, const uint16_t* Ah, const uint16_t* Al, const double* x,
-                   double* y, double alpha, double beta) {
+static int gemv_lp(Ctx* c, int prec, const float* A32, const uint16_t* Ab, const double* x, double* y,
+                   double alpha, double beta) {
   const Dims& d = c->d;
   const int rows_per_block = 4;
   const bool f32 = prec == CCSD_PREC_FP32;
   hipLaunchKernelGGL(gemv_lp_kernel, dim3((unsigned)((d.ov + rows_per_block - 1) / rows_per_block)),
-                     dim3(64 * rows_per_block), 0, c->stream, y, f32 ? A32 : nullptr, f32 ? nullptr : Ah,
-                     f32 ? nullptr : Al, c->lp->ld_ov, x, d.ov, d.ov, alpha, beta);
+                     dim3(64 * rows_per_block), 0, c->stream, y, f32 ? A32 : nullptr, f32 ? nullptr : Ab,
+                     c->lp->ld_ov, x, d.ov, d.ov, alpha, beta);
   return CCSD_OK;
 }
 
 int lowprec_fme(Ctx* c, int prec, const double* t1) {
   LowPrec* L = c->lp;
-  return gemv_lp(c, prec, L->Doovv32, L->Doovv_h, L->Doovv_l, t1, c->Fme, 1.0, 1.0);
+  return gemv_lp(c, prec, L->Doovv32, L->Doovv_b, t1, c->Fme, 1.0, 1.0);
 }
 
 int lowprec_r1_ring(Ctx* c, int prec) {
   LowPrec* L = c->lp;
-  return gemv_lp(c, prec, L->Tring32, L->Tring_h, L->Tring_l, c->Fme, c->R1, 1.0, 1.0);
+  return gemv_lp(c, prec, L->Tring32, L->Tring_b, c->Fme, c->R1, 1.0, 1.0);
 }
 
-// C64 (M x N, ldc) = alpha * A B^T + beta * C64, operands as stored low-precision
-// pairs; in BF16 mode the product can also be emitted as a hi/lo pair (Oh/Ol, ldo).
-static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ah, const uint16_t* Al, long lda,
-                      const float* B32, const uint16_t* Bh, const uint16_t* Bl, long ldb, int M, int N, int K,
-                      double* C, long ldc, double alpha, double beta, uint16_t* Oh = nullptr,
-                      uint16_t* Ol = nullptr, long ldo = 0) {
+// C64 (M x N, ldc) = alpha * A B^T + beta * C64 on the stored low-precision
+// operands; in BF16 mode the product can also be emitted as an interleaved
+// hi/lo operand (Ob, leading dimension ldo).
+static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, long lda, const float* B32,
+                      const uint16_t* Bb, long ldb, int M, int N, int K, double* C, long ldc, double alpha,
+                      double beta, uint16_t* Ob = nullptr, long ldo = 0) {
   hipError_t e;
   if (prec == CCSD_PREC_FP32) {
     GemmDesc<float> g;
@@ -603,8 +510,8 @@ static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ah, co
   } else {
     Bf16Out o;
     o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
-    o.Oh = Oh; o.Ol = Ol; o.ldo = ldo;
-    e = bf16x3_nt(Ah, Al, lda, Bh, Bl, ldb, o, M, N, K, c->stream);
+    o.Ob = Ob; o.ldo = ldo;
+    e = bf16x3_nt(Ab, lda, Bb, ldb, o, M, N, K, c->stream);
   }
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
@@ -616,8 +523,8 @@ static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ah, co
 int lowprec_w2(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
-  return nt_product(c, prec, L->tau32, L->tau_h, L->tau_l, L->ld_pv, L->ovvv32, L->ovvv_h, L->ovvv_l, L->ld_pv,
-                    (int)d.po, (int)d.ov, (int)d.pv, c->W2, d.ov, 1.0, 1.0);
+  return nt_product(c, prec, L->tau32, L->tau_b, L->ld_pv, L->ovvv32, L->ovvv_b, L->ld_pv, (int)d.po, (int)d.ov,
+                    (int)d.pv, c->W2, d.ov, 1.0, 1.0);
 }
 
 int lowprec_wmbej(Ctx* c, int prec) {
@@ -625,25 +532,27 @@ int lowprec_wmbej(Ctx* c, int prec) {
   const Dims& d = c->d;
   // Doovv is symmetric, so its rows serve as the k-contiguous B operand.  In BF16
   // mode the finished W_mbej is also written as the ring GEMM's hi/lo operand.
-  return nt_product(c, prec, L->Tring32, L->Tring_h, L->Tring_l, L->ld_ov, L->Doovv32, L->Doovv_h, L->Doovv_l,
-                    L->ld_ov, (int)d.ov, (int)d.ov, (int)d.ov, c->WT, d.ov, 0.5, 1.0, L->WT_h, L->WT_l, L->ld_ov);
+  return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->Doovv32, L->Doovv_b, L->ld_ov, (int)d.ov,
+                    (int)d.ov, (int)d.ov, c->WT, d.ov, 0.5, 1.0, L->WT_b, L->ld_ov);
 }
 
 int lowprec_ladder(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
   // <ef||ab> = <ab||ef>: the packed vvvv block is symmetric, rows are k-contiguous B.
-  return nt_product(c, prec, L->tau32, L->tau_h, L->tau_l, L->ld_pv, L->vvvv32, L->vvvv_h, L->vvvv_l, L->ld_pv,
-                    (int)d.po, (int)d.pv, (int)d.pv, c->Rp, d.pv, 1.0, 1.0);
+  return nt_product(c, prec, L->tau32, L->tau_b, L->ld_pv, L->vvvv32, L->vvvv_b, L->ld_pv, (int)d.po, (int)d.pv,
+                    (int)d.pv, c->Rp, d.pv, 1.0, 1.0);
 }
 
 int lowprec_ring(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
-  // BF16: WT_h/WT_l come from the W_mbej epilogue (WT is final there)
-  if (prec == CCSD_PREC_FP32) conv(c, prec, c->WT, d.ov, d.ov, L->ld_ov, L->WT32, nullptr, nullptr);
-  return nt_product(c, prec, L->Tring32, L->Tring_h, L->Tring_l, L->ld_ov, L->WT32, L->WT_h, L->WT_l, L->ld_ov,
-                    (int)d.ov, (int)d.ov, (int)d.ov, c->X, d.ov, 1.0, 0.0);
+  // BF16: WT_b comes from the W_mbej epilogue (WT is final there)
+  if (prec == CCSD_PREC_FP32)
+    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, c->stream, L->WT32,
+                       c->WT, d.ov, d.ov, d.ov, L->ld_ov, 1.0);
+  return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->WT32, L->WT_b, L->ld_ov, (int)d.ov, (int)d.ov,
+                    (int)d.ov, c->X, d.ov, 1.0, 0.0);
 }
 
 GemmDesc<float> f32_desc(const GemmDesc<double>& g, const float* A32, const float* B32) {
@@ -664,7 +573,7 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
   prec &= 0xff;
   const long ld = pad64(K);
   float *A32 = nullptr, *B32 = nullptr;
-  uint16_t *Ah = nullptr, *Al = nullptr, *Bh = nullptr, *Bl = nullptr;
+  uint16_t *Ab = nullptr, *Bb = nullptr;
   std::vector<void*> owned;
   auto al = [&](void** p, size_t b) {
     hipError_t e = hipMalloc(p, std::max<size_t>(b, 16));
@@ -676,10 +585,8 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
     e = al((void**)&A32, (size_t)M * ld * 4);
     if (e == hipSuccess) e = al((void**)&B32, (size_t)N * ld * 4);
   } else {
-    e = al((void**)&Ah, (size_t)M * ld * 2);
-    if (e == hipSuccess) e = al((void**)&Al, (size_t)M * ld * 2);
-    if (e == hipSuccess) e = al((void**)&Bh, (size_t)N * ld * 2);
-    if (e == hipSuccess) e = al((void**)&Bl, (size_t)N * ld * 2);
+    e = al((void**)&Ab, (size_t)M * ld * 4);
+    if (e == hipSuccess) e = al((void**)&Bb, (size_t)N * ld * 4);
   }
   void* wsp = nullptr;
   if (e == hipSuccess) e = al(&wsp, 16u << 20);
@@ -697,13 +604,13 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
       GemmWorkspace ws{wsp, 16u << 20};
       e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
     } else {
-      hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(M * ld)), dim3(kThreads), 0, s, Ah, Al, A, (long)M,
+      hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(M * ld / 8)), dim3(kThreads), 0, s, Ab, A, (long)M,
                          (long)K, (long)K, ld, 1.0);
-      hipLaunchKernelGGL(conv_bf16x2_kernel, dim3(grid_for(N * ld)), dim3(kThreads), 0, s, Bh, Bl, B, (long)N,
+      hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(N * ld / 8)), dim3(kThreads), 0, s, Bb, B, (long)N,
                          (long)K, (long)K, ld, 1.0);
       Bf16Out o;
       o.C64 = C; o.ldc = N; o.alpha = alpha; o.beta = beta;
-      e = bf16x3_nt(Ah, Al, ld, Bh, Bl, ld, o, M, N, K, s, tile - 1);
+      e = bf16x3_nt(Ab, ld, Bb, ld, o, M, N, K, s, tile - 1);
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);

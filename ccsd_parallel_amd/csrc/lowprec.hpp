@@ -3,7 +3,9 @@
 // The four dominant contractions (pp ladder, ring, W_mbej quadratic term,
 // W_mbij tau term) run as FP32 MFMA GEMMs or BF16 hi/lo x3 GEMMs on operands
 // stored k-contiguous with a leading dimension padded to a multiple of 64
-// (zero filled), so those kernels need no K bounds checks.  Every other
+// (zero filled), so those kernels need no K bounds checks; BF16 operands keep
+// hi and lo interleaved per 8-element chunk (one 128-byte line per row per
+// 32-element k-tile).  Every other
 // contraction of the residual runs on FP32 copies of its operands (same
 // layouts as the FP64 arrays) through the generic MFMA engine.  All products
 // accumulate in FP32 and land in the FP64 intermediates directly (fused
@@ -22,11 +24,9 @@ struct LowPrec {
   // and tau32 also serve the secondary contractions in both modes
   float *vvvv32 = nullptr, *ovvv32 = nullptr, *Doovv32 = nullptr;
   float *tau32 = nullptr, *Tring32 = nullptr, *WT32 = nullptr;
-  // BF16 hi/lo operands
-  uint16_t *vvvv_h = nullptr, *vvvv_l = nullptr, *ovvv_h = nullptr, *ovvv_l = nullptr;
-  uint16_t *Doovv_h = nullptr, *Doovv_l = nullptr;
-  uint16_t *tau_h = nullptr, *tau_l = nullptr, *Tring_h = nullptr, *Tring_l = nullptr, *WT_h = nullptr,
-           *WT_l = nullptr;
+  // BF16 hi/lo operands, interleaved per 8-element chunk (row r at r * 2 * ld)
+  uint16_t *vvvv_b = nullptr, *ovvv_b = nullptr, *Doovv_b = nullptr;
+  uint16_t *tau_b = nullptr, *Tring_b = nullptr, *WT_b = nullptr;
   // FP32 copies for the secondary contractions, same layouts as the FP64 arrays:
   // static integrals ...
   float *oovv_f = nullptr, *oovv_vu_f = nullptr, *oovv_ou_f = nullptr, *G_f = nullptr, *ovvv_u_f = nullptr;
