@@ -55,7 +55,7 @@ enum ccsd_precision {
 
 typedef struct ccsd_options {
   int precision;            /* ccsd_precision used before the switch to FP64 */
-  double switch_threshold;  /* mixed modes: switch to FP64 once |dE| < max(this, tol) (default 0: at tol) */
+  double switch_threshold;  /* mixed modes: switch to FP64 once |dE| < max(this, switch_factor * tol, tol) (default 0) */
   int final_fp64_sweeps;    /* minimum FP64 iterations after the switch (default 3) */
   int diis_dim;             /* 0 or 1: plain Jacobi as the reference; >= 2: DIIS subspace (default 8) */
   int diis_start;           /* first iteration allowed to extrapolate (default 1) */
@@ -64,6 +64,8 @@ typedef struct ccsd_options {
   int device;               /* HIP device ordinal */
   size_t hbm_budget_bytes;  /* 0 = 90 % of the device's free memory */
   int diis_reset_on_switch; /* mixed modes: clear the DIIS subspace at the switch to FP64 (default 1) */
+  double switch_factor;     /* mixed modes: switch once |dE| < switch_factor * tol (default 100: the FP64
+                               sweeps then take |dE| down the last two decades) */
 } ccsd_options;
 
 typedef struct ccsd_problem {

@@ -84,6 +84,7 @@ def test_strerror_and_defaults():
         assert capi.strerror(code) and capi.strerror(code) != "unknown error"
     o = capi.default_options()
     assert o.diis_dim == 8 and o.final_fp64_sweeps == 3 and o.precision == 0
+    assert o.switch_factor == 100.0 and o.switch_threshold == 0.0 and o.diis_reset_on_switch == 1
 
 
 def test_plan_fits_hbm():
