@@ -11,6 +11,8 @@ Workload: O=40 occupied / V=300 virtual spin orbitals, seed 40300, FP64.
   * per-iteration cost in each precision;
   * pp-ladder TF/s and its roofline fraction (HIP events on the engine stream,
     averaged over the timed steps);
+  * mixed forward + adjoint against FP64 forward + adjoint (wall time, |dE|,
+    gradient relative error);
   * the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
@@ -97,6 +99,43 @@ def ladder_traffic(O, V, precision):
     return None
 
 
+def combined_forward_adjoint(capi, O, V, seed, scale, tol=1e-7):
+    """Mixed forward + adjoint against FP64 forward + adjoint on the same
+    workload: wall time of solve (to |dE| < tol) plus backward (Lambda
+    equations and the density contractions into dE/df, dE/d<pq||rs>, written
+    into page-locked host arrays), |dE_corr| and the relative error of the
+    whole gradient (Frobenius norm over all blocks)."""
+    res, bufs = {}, {}
+    try:
+        for p in ("fp64", "bf16x3"):
+            c = capi.Context.synthetic(O, V, seed, scale, precision=p, diis_dim=8)
+            bufs[p] = capi.PinnedArrays(c.gradient_shapes())
+            t0 = time.perf_counter()
+            r = c.solve(tol, 100)
+            t1 = time.perf_counter()
+            _, info = c.backward(out=bufs[p].arrays)
+            t2 = time.perf_counter()
+            c.close()
+            res[p] = {"e_corr": r["e_corr"], "forward_s": t1 - t0, "backward_s": t2 - t1, "total_s": t2 - t0,
+                      "forward_iterations": f"{r['lowp_iterations']} low + {r['fp64_iterations']} FP64",
+                      "lambda_iterations": f"{info['lambda_lowp_iterations']} low + "
+                                           f"{info['lambda_iterations'] - info['lambda_lowp_iterations']} FP64"}
+        a, b = bufs["fp64"].arrays, bufs["bf16x3"].arrays
+        num = den = 0.0
+        for k in a:
+            for r0 in range(0, a[k].shape[0], 1024):  # row chunks: no full-size temporaries
+                x, y = a[k][r0:r0 + 1024], b[k][r0:r0 + 1024]
+                num += float(np.sum((y - x) ** 2))
+                den += float(np.sum(x * x))
+        res["abs_dE_corr"] = abs(res["bf16x3"]["e_corr"] - res["fp64"]["e_corr"])
+        res["gradient_rel_error"] = float(np.sqrt(num / den))
+        res["speedup_total"] = res["fp64"]["total_s"] / res["bf16x3"]["total_s"]
+    finally:
+        for bb in bufs.values():
+            bb.close()
+    return res
+
+
 def timed_steps(ctx, steps, warmup, precision):
     """Wall time per step and HIP-event kernel times averaged over the timed steps."""
     ctx.iterate(warmup, precision)
@@ -122,6 +161,7 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU; the multi-GPU sharding is a design note (DESIGN.md section 7)",
@@ -185,6 +225,8 @@ def main():
             conv[p]["abs_dE_vs_fp64"] = abs(conv[p]["e_corr"] - conv["fp64"]["e_corr"])
             conv[p]["speedup_vs_fp64"] = conv["fp64"]["seconds_total"] / conv[p]["seconds_total"]
         out["time_to_converge_1e-7"] = conv
+    if not args.no_adjoint:
+        out["forward_adjoint_1e-7"] = combined_forward_adjoint(capi, O, V, seed, scale)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(O, V, seed, scale)
     print(json.dumps(out))

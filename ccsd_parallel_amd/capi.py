@@ -24,6 +24,7 @@ EXPORTED = (
     "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
     "ccsd_selftest_gemm", "ccsd_update_integrals",
+    "ccsd_host_alloc", "ccsd_host_free",
     # include/ccsd_rhf.h (host RHF input producer)
     "ccsd_basis_size", "ccsd_rhf", "ccsd_ao_one_electron",
 )
@@ -58,7 +59,7 @@ class Result(C.Structure):
 
 class GradInfo(C.Structure):
     _fields_ = [("lambda_iterations", C.c_int), ("converged", C.c_int), ("lambda_residual", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("lambda_lowp_iterations", C.c_int)]
 
 
 class RhfResult(C.Structure):
@@ -108,6 +109,8 @@ def load():
     lib.ccsd_update_integrals.argtypes = [vp, C.c_int, C.c_long, C.POINTER(C.c_long), dp]
     lib.ccsd_selftest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_double,
                                        dp, C.c_int]
+    lib.ccsd_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    lib.ccsd_host_free.argtypes = [vp]
     ip = C.POINTER(C.c_int)
     lib.ccsd_basis_size.argtypes = [C.c_int, ip, C.c_char_p]
     lib.ccsd_rhf.argtypes = [C.c_int, ip, dp, C.c_char_p, C.c_int, C.c_double, C.POINTER(RhfResult), dp, dp, dp, dp,
@@ -260,12 +263,24 @@ class Context:
                                               idx.ctypes.data_as(C.POINTER(C.c_long)), _ptr(dl)),
                "ccsd_update_integrals")
 
-    def backward(self, allow_unconverged=False):
-        O, V = self.O, self.V
-        N = O + V
-        g = dict(f=np.zeros((N, N)), oooo=np.zeros((self.po, self.po)), ooov=np.zeros((self.po, self.ov)),
-                 oovv=np.zeros((self.po, self.pv)), ovov=np.zeros((self.ov, self.ov)),
-                 ovvv=np.zeros((self.ov, self.pv)), vvvv=np.zeros((self.pv, self.pv)))
+    def gradient_shapes(self):
+        N = self.O + self.V
+        return dict(f=(N, N), oooo=(self.po, self.po), ooov=(self.po, self.ov), oovv=(self.po, self.pv),
+                    ovov=(self.ov, self.ov), ovvv=(self.ov, self.pv), vvvv=(self.pv, self.pv))
+
+    def backward(self, allow_unconverged=False, out=None):
+        """dE/df and dE/d<pq||rs> (dict of arrays in the packed layouts) and the
+        Lambda statistics.  ``out`` may supply the destination arrays (C-contiguous
+        float64 of gradient_shapes(), e.g. page-locked for DMA-rate copies)."""
+        shapes = self.gradient_shapes()
+        if out is None:
+            g = {k: np.zeros(sh) for k, sh in shapes.items()}
+        else:
+            g = out
+            for k, sh in shapes.items():
+                a = g[k]
+                if a.shape != sh or a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+                    raise ValueError(f"out[{k!r}] must be a C-contiguous float64 array of shape {sh}")
         info = GradInfo()
         rc = self.lib.ccsd_backward(self.h, *[_ptr(g[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv",
                                                                     "vvvv")], C.byref(info))
@@ -342,3 +357,34 @@ def ao_one_electron(Z, R, basis, angstrom=False):
     _check(load().ccsd_ao_one_electron(len(Zc), Zc.ctypes.data_as(C.POINTER(C.c_int)), _ptr(Rc), basis.encode(),
                                        _ptr(S), _ptr(T), _ptr(V), n), "ccsd_ao_one_electron")
     return S, T, V
+
+
+class PinnedArrays:
+    """Page-locked float64 host arrays (ccsd_host_alloc), e.g. destinations for
+    Context.backward(out=...); freed by close() or on exit of a with-block."""
+
+    def __init__(self, shapes):
+        self.arrays, self._ptrs = {}, []
+        lib = load()
+        try:
+            for k, sh in shapes.items():
+                n = int(np.prod(sh))
+                p = C.c_void_p()
+                _check(lib.ccsd_host_alloc(n * 8, C.byref(p)), "ccsd_host_alloc")
+                self._ptrs.append(p)
+                self.arrays[k] = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n,)).reshape(sh)
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        self.arrays = {}
+        for p in self._ptrs:
+            load().ccsd_host_free(p)
+        self._ptrs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

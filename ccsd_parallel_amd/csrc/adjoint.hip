@@ -13,6 +13,11 @@
 //  The four-virtual block gradient (P2(V) x P2(V), 16 GB at O=40 V=300) is
 //  never held whole in HBM: it is recomputed in row slabs from tau and Lambda
 //  and streamed to the host (opt.recompute_v4).
+//  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
+//    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
+//    until max|dLambda| < switch_factor * tol, then at least
+//    final_fp64_sweeps FP64 iterations -- the forward's schedule, with the
+//    DIIS subspace kept across the switch.  The gradient pass is always FP64.
 // The reference has no adjoint; its forward algebra is restated in engine.hip.
 #include <algorithm>
 #include <chrono>
@@ -21,6 +26,7 @@
 #include <vector>
 
 #include "adjoint.hpp"
+#include "lowprec.hpp"
 
 namespace ccsd {
 
@@ -369,7 +375,7 @@ struct ThetaBars {
 // at the converged amplitudes) with cotangents (lam1, lam2).  Writes
 // (dG/dT)^T lam into A->tb; with th != nullptr also accumulates the integral
 // and Fock cotangents.
-static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th) {
+static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = false) {
   AdjState* A = c->adj;
   const Dims& d = c->d;
   const long O = d.O, V = d.V, VV = V * V, po = d.po, pv = d.pv, ov = d.ov;
@@ -397,7 +403,7 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th) {
   zero(A->WTb, ov * ov);
 
   // -- assembly: R2 = Rp + P(ij)P(ab) X + P(ab) Y + P(ij) Z
-  k_unpack_ring(A->Xb, lam2, d, s);
+  if (!lowp) k_unpack_ring(A->Xb, lam2, d, s);  // low precision: unpacked into the BF16 operand below
   k_unpack_vir(A->Yb, lam2, d, s);
   k_unpack_occ(A->Zb, A->Zb, lam2, lam2, d, s);
 
@@ -468,11 +474,16 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th) {
   }
 
   // -- ring X = Tring . WT^T
-  TRY(run_gemm(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(A->Xb, 1, ov, c->Tring, ov, 1, A->WTb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
+  if (lowp) {
+    TRY(lowprec_adj_ring(c, A->Tringb, A->WTb, lam2));
+  } else {
+    TRY(run_gemm(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
+    TRY(run_gemm(c, G64(A->Xb, 1, ov, c->Tring, ov, 1, A->WTb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
+  }
 
   // -- Rp = <ab||ij> + tau . vvvv + W1^T . tau     (cotangent lam2)
-  TRY(run_gemm(c, G64(lam2, pv, 1, c->vvvv, pv, 1, A->taub, pv, 1, po, pv, pv, 1.0, 1.0), W));
+  if (lowp) TRY(lowprec_adj_ladder(c, A->taub, lam2));
+  else TRY(run_gemm(c, G64(lam2, pv, 1, c->vvvv, pv, 1, A->taub, pv, 1, po, pv, pv, 1.0, 1.0), W));
   TRY(run_gemm(c, G64(c->tau, pv, 1, lam2, 1, pv, A->W1b, po, 1, po, po, pv, 1.0, 1.0), W));
   TRY(run_gemm(c, G64(c->W1, po, 1, lam2, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), W));
   if (th) LAUNCH(scale_add_kernel, po * pv, th->oovv, lam2, 1.0, po * pv);
@@ -498,7 +509,8 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th) {
   }
 
   // -- W_mbej (WT): (c) + 1/2 Tring . Doovv
-  TRY(run_gemm(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), W));
+  if (lowp) TRY(lowprec_adj_wmbej(c, A->Tringb, A->WTb));
+  else TRY(run_gemm(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), W));
   if (th) TRY(run_gemm(c, G64(c->Tring, ov, 1, A->WTb, ov, 1, th->Doovv, ov, 1, ov, ov, ov, 0.5, 1.0), W));
   // (b) + t1[n,b] W3u[j][m][n][e], batched over (j, m)
   {
@@ -534,7 +546,8 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th) {
     g.sB1 = V; g.sB2 = ov; g.sC1 = V * ov; g.sC2 = V;
     TRY(run_gemm(c, g, W));
   }
-  TRY(run_gemm(c, G64(A->W2b, ov, 1, c->ovvv, pv, 1, A->taub, pv, 1, po, pv, ov, 1.0, 1.0), W));
+  if (lowp) TRY(lowprec_adj_w2(c, A->taub, A->W2b));
+  else TRY(run_gemm(c, G64(A->W2b, ov, 1, c->ovvv, pv, 1, A->taub, pv, 1, po, pv, ov, 1.0, 1.0), W));
   if (th) {
     TRY(run_gemm(c, G64(A->W2b, 1, ov, c->tau, pv, 1, th->ovvv, pv, 1, ov, pv, po, 1.0, 1.0), W));
     LAUNCH(scale_add_kernel, po * ov, th->ooov, A->W2b, 1.0, po * ov);
@@ -610,16 +623,23 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   HIPTRY(hipMemcpyAsync(A->e, c->fov, ov * 8, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(tau_vjp_t1_kernel, dim3(ov), dim3(kThreads), 0, s, A->e, c->oovv, t1, d);
 
-  // Lambda iterations (Jacobi + DIIS on the forward's DIIS buffers)
+  // Lambda iterations (Jacobi + DIIS on the forward's DIIS buffers); BF16x3
+  // reverse products until the switch, then FP64 sweeps
   LAUNCH(zero_kernel, c->nvec, A->lam, c->nvec);
   const int m = c->diis_m;
   std::vector<double> B((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
-  int count = 0, next = 0, it = 0;
+  int count = 0, next = 0, it = 0, lowp_its = 0, fp64_after = c->opt.final_fp64_sweeps;
   double delta = 1e300;
   const double tol = 1e-10;
-  bool conv = false;
+  bool conv = false, lowp = c->opt.precision == CCSD_PREC_BF16X3;
+  if (lowp) {
+    TRY(lowprec_adj_setup(c));
+    fp64_after = 0;
+  }
   for (it = 1; it <= 500; ++it) {
-    TRY(reverse_pass(c, A->lam, nullptr));
+    TRY(reverse_pass(c, A->lam, nullptr, lowp));
+    if (lowp) ++lowp_its;
+    else ++fp64_after;
     LAUNCH(lambda_update_kernel, c->nvec, A->lnew, A->lerr, A->e, A->tb, A->lam, c->f, d);
     TRY(maxabs(c, A->lerr, c->nvec, &delta));
     if (!std::isfinite(delta)) return CCSD_ERR_NONFINITE;
@@ -649,7 +669,14 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
       }
     }
     if (!extrap) std::swap(A->lam, A->lnew);
-    if (delta < tol) { conv = true; break; }
+    if (lowp) {
+      // refine in FP64 from here on; the DIIS subspace is kept (at O=40/V=300:
+      // 15 + 5 iterations against 15 + 7 with a reset), since the tight
+      // max|dLambda| < 1e-10 test on FP64 updates pins the FP64 fixed point
+      if (delta < std::max(c->opt.switch_factor * tol, tol)) lowp = false;
+      continue;
+    }
+    if (delta < tol && fp64_after >= c->opt.final_fp64_sweeps) { conv = true; break; }
   }
 
   // gradient pass
@@ -694,16 +721,70 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   hipLaunchKernelGGL(fdiag_kernel, dim3(N), dim3(kThreads), 0, s, th.f, A->lam, c->amp, d);
   if (hipGetLastError() != hipSuccess) { cleanup(); return CCSD_ERR_HIP; }
 
-  // copy out (small blocks)
-  auto d2h = [&](double* h, const double* dv, long n) {
-    return hipMemcpyAsync(h, dv, (size_t)n * 8, hipMemcpyDeviceToHost, s) == hipSuccess;
+  // Copies to the host go on a second stream (default priority) so they overlap
+  // the four-virtual slab GEMMs; page-locked destination buffers make them DMA
+  // transfers.  The derived-layout cotangents stay allocated until the end.
+  hipStream_t cs = nullptr;
+  hipEvent_t ev_fold = nullptr, ev_comp[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+  double* slab[2] = {nullptr, nullptr};
+  auto teardown = [&]() {
+    if (cs) (void)hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(s);
+    for (double* p : slab)
+      if (p) (void)hipFree(p);
+    for (hipEvent_t e : {ev_fold, ev_comp[0], ev_comp[1], ev_copy[0], ev_copy[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (cs) (void)hipStreamDestroy(cs);
+    cleanup();
   };
+  bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&ev_fold, hipEventDisableTiming) == hipSuccess;
+  for (int b = 0; b < 2 && ok; ++b)
+    ok = hipEventCreateWithFlags(&ev_comp[b], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ev_copy[b], hipEventDisableTiming) == hipSuccess;
   std::vector<double> fov(ov), fvv(VV), fmi(O * O), fd(N);
-  bool ok = d2h(g.oooo, th.oooo, po * po) && d2h(g.ooov, th.ooov, po * ov) && d2h(g.oovv, th.oovv, po * pv) &&
-            d2h(g.ovov, ovovb, ov * ov) && d2h(g.ovvv, th.ovvv, ov * pv) && d2h(fov.data(), th.fov, ov) &&
-            d2h(fvv.data(), th.fvv, VV) && d2h(fmi.data(), A->Fmib, O * O) && d2h(fd.data(), th.f, N);
-  if (!ok || hipStreamSynchronize(s) != hipSuccess) { cleanup(); return CCSD_ERR_HIP; }
-  cleanup();
+  auto d2h = [&](double* h, const double* dv, long n) {
+    return hipMemcpyAsync(h, dv, (size_t)n * 8, hipMemcpyDeviceToHost, cs) == hipSuccess;
+  };
+  ok = ok && hipEventRecord(ev_fold, s) == hipSuccess && hipStreamWaitEvent(cs, ev_fold, 0) == hipSuccess;
+  ok = ok && d2h(g.oooo, th.oooo, po * po) && d2h(g.ooov, th.ooov, po * ov) && d2h(g.oovv, th.oovv, po * pv) &&
+       d2h(g.ovov, ovovb, ov * ov) && d2h(g.ovvv, th.ovvv, ov * pv) && d2h(fov.data(), th.fov, ov) &&
+       d2h(fvv.data(), th.fvv, VV) && d2h(fmi.data(), A->Fmib, O * O) && d2h(fd.data(), th.f, N);
+  if (!ok) {
+    teardown();
+    return CCSD_ERR_HIP;
+  }
+
+  // four-virtual block: dE/d<ef||ab> = 1/2 (tau^T lam2 + lam2^T tau), in row slabs
+  // computed into two alternating device buffers and streamed to the host
+  {
+    const double* lam2 = A->lam + ov;
+    const long slab_rows = std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv));
+    for (int b = 0; b < 2; ++b)
+      if (hipMalloc(&slab[b], (size_t)slab_rows * pv * 8) != hipSuccess) {
+        teardown();
+        return CCSD_ERR_HIP;
+      }
+    long k = 0;
+    for (long r0 = 0; r0 < pv && rc == CCSD_OK; r0 += slab_rows, ++k) {
+      const long nr = std::min(slab_rows, pv - r0);
+      const int b = (int)(k & 1);
+      if (k >= 2 && hipStreamWaitEvent(s, ev_copy[b], 0) != hipSuccess) rc = CCSD_ERR_HIP;
+      if (rc == CCSD_OK)
+        rc = run_gemm(c, G64(c->tau + r0, 1, pv, lam2, pv, 1, slab[b], pv, 1, nr, pv, po, 0.5, 0.0), &A->ws);
+      if (rc == CCSD_OK)
+        rc = run_gemm(c, G64(lam2 + r0, 1, pv, c->tau, pv, 1, slab[b], pv, 1, nr, pv, po, 0.5, 1.0), &A->ws);
+      if (rc == CCSD_OK &&
+          (hipEventRecord(ev_comp[b], s) != hipSuccess || hipStreamWaitEvent(cs, ev_comp[b], 0) != hipSuccess ||
+           hipMemcpyAsync(g.vvvv + r0 * pv, slab[b], (size_t)nr * pv * 8, hipMemcpyDeviceToHost, cs) != hipSuccess ||
+           hipEventRecord(ev_copy[b], cs) != hipSuccess))
+        rc = CCSD_ERR_HIP;
+    }
+    if (rc == CCSD_OK && (hipStreamSynchronize(cs) != hipSuccess || hipStreamSynchronize(s) != hipSuccess))
+      rc = CCSD_ERR_HIP;
+    teardown();
+    if (rc != CCSD_OK) return rc;
+  }
 
   // Fock cotangent: ov block (used as f[i][O+a]), oo / vv off-diagonal, diagonal from D; symmetrised
   std::vector<double> G((size_t)N * N, 0.0);
@@ -718,29 +799,9 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   for (long p = 0; p < N; ++p) G[p * N + p] = fd[p];
   for (long p = 0; p < N; ++p)
     for (long q = 0; q < N; ++q) g.fock[p * N + q] = 0.5 * (G[p * N + q] + G[q * N + p]);
-
-  // four-virtual block: dE/d<ef||ab> = 1/2 (tau^T lam2 + lam2^T tau), in row slabs streamed to the host
-  {
-    const double* lam2 = A->lam + ov;
-    long slab_rows = std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv));
-    double* slab = nullptr;
-    if (hipMalloc(&slab, (size_t)slab_rows * pv * 8) != hipSuccess) return CCSD_ERR_HIP;
-    for (long r0 = 0; r0 < pv; r0 += slab_rows) {
-      const long nr = std::min(slab_rows, pv - r0);
-      rc = run_gemm(c, G64(c->tau + r0, 1, pv, lam2, pv, 1, slab, pv, 1, nr, pv, po, 0.5, 0.0), &A->ws);
-      if (rc == CCSD_OK) rc = run_gemm(c, G64(lam2 + r0, 1, pv, c->tau, pv, 1, slab, pv, 1, nr, pv, po, 0.5, 1.0), &A->ws);
-      if (rc != CCSD_OK) break;
-      if (hipMemcpyAsync(g.vvvv + r0 * pv, slab, (size_t)nr * pv * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess) {
-        rc = CCSD_ERR_HIP;
-        break;
-      }
-    }
-    (void)hipFree(slab);
-    if (rc != CCSD_OK) return rc;
-  }
   if (info) {
-    info->lambda_iterations = it;
+    info->lambda_iterations = std::min(it, 500);
+    info->lambda_lowp_iterations = lowp_its;
     info->converged = conv ? 1 : 0;
     info->lambda_residual = delta;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -255,6 +255,19 @@ int ccsd_default_options(ccsd_options* o) {
   return CCSD_OK;
 }
 
+int ccsd_host_alloc(size_t bytes, void** out) {
+  if (!out || bytes == 0) return CCSD_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCSD_ERR_NO_DEVICE;
+  return hipHostMalloc(out, bytes, hipHostMallocDefault) == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+}
+
+int ccsd_host_free(void* p) {
+  if (!p) return CCSD_OK;
+  return hipHostFree(p) == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+}
+
 size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision) {
   if (n_occ < 2 || n_virt < 2) return 0;
   return plan_bytes(n_occ, n_virt, precision, 8);

@@ -67,6 +67,31 @@ __global__ void conv_bf16il_kernel(uint16_t* dst, const double* src, long rows, 
   }
 }
 
+// dst (interleaved, rows x ld) = split(src[k*sld + r]): the transpose of a
+// row-major source; threads run along r so the source reads coalesce.
+__global__ void conv_bf16il_t_kernel(uint16_t* dst, const double* src, long rows, long K, long sld, long ld) {
+  const long chunks = ld / 8, n = rows * chunks;
+  GRID_LOOP(idx, n) {
+    const long c = idx / rows, r = idx % rows;
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long k = c * 8 + e;
+      h[e] = 0;
+      l[e] = 0;
+      if (k < K) bf16_split(src[k * sld + r], h[e], l[e]);
+    }
+    uint4 hv, lv;
+    hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
+    hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
+    lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
+    lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
+    uint4* o = (uint4*)(dst + r * 2 * ld + c * 16);
+    o[0] = hv;
+    o[1] = lv;
+  }
+}
+
 // y[m] = alpha * sum_k A[m][k] x[k] + beta * y[m], A given as FP32 (A32, row
 // stride ld) or as interleaved BF16 hi/lo rows (Ab); one wavefront per row,
 // 8 consecutive k per lane (16-byte loads) and a scalar tail.
@@ -553,6 +578,77 @@ int lowprec_ring(Ctx* c, int prec) {
                        c->WT, d.ov, d.ov, d.ov, L->ld_ov, 1.0);
   return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->WT32, L->WT_b, L->ld_ov, (int)d.ov, (int)d.ov,
                     (int)d.ov, c->X, d.ov, 1.0, 0.0);
+}
+
+// ---------------------------------------------------------------- adjoint (Lambda) pieces
+static int bf16_nt(Ctx* c, const uint16_t* A, long lda, const uint16_t* B, long ldb, long M, long N, long K,
+                   double* C, long ldc, double alpha, double beta) {
+  Bf16Out o;
+  o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
+  hipError_t e = bf16x3_nt(A, lda, B, ldb, o, (int)M, (int)N, (int)K, c->stream);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  return CCSD_OK;
+}
+
+int lowprec_adj_setup(Ctx* c) {
+  TRY(lowprec_allocate(c, CCSD_PREC_BF16X3));
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  if (!L->have_adj) {
+    TRY(lp_alloc(c, &L->WTt_b, 2 * d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->WTbar_b, 2 * d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->W2bar_b, 2 * d.po * L->ld_ov));
+    L->have_adj = 1;
+  }
+  if (!L->have_ovvvT) {  // <ia||bc>^T: rows (b<c), k = (i,a); static
+    TRY(lp_alloc(c, &L->ovvvT_b, 2 * d.pv * L->ld_ov));
+    hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.pv * L->ld_ov / 8)), dim3(kThreads), 0, s, L->ovvvT_b,
+                       c->ovvv, d.pv, d.ov, d.pv, L->ld_ov);
+    L->have_ovvvT = 1;
+  }
+  // fixed at the converged amplitudes: W_mbej^T and Tring (FP64 forward intermediates)
+  hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.ov * L->ld_ov / 8)), dim3(kThreads), 0, s, L->WTt_b,
+                     c->WT, d.ov, d.ov, d.ov, L->ld_ov);
+  conv_il(c, L->Tring_b, c->Tring, d.ov, d.ov, L->ld_ov);
+  HIPTRY(hipGetLastError());
+  return CCSD_OK;
+}
+
+int lowprec_adj_ring(Ctx* c, double* Tringb, double* WTb, const double* lam2) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  // Xb (symmetric) as a BF16 operand, in the forward's WT_b buffer (rebuilt by every forward step)
+  RingOut r;
+  r.b = L->WT_b;
+  r.ld = L->ld_ov;
+  k_unpack_vir_ring_lp(nullptr, r, lam2, d, c->stream);
+  TRY(bf16_nt(c, L->WT_b, L->ld_ov, L->WTt_b, L->ld_ov, d.ov, d.ov, d.ov, Tringb, d.ov, 1.0, 1.0));
+  return bf16_nt(c, L->WT_b, L->ld_ov, L->Tring_b, L->ld_ov, d.ov, d.ov, d.ov, WTb, d.ov, 1.0, 1.0);
+}
+
+int lowprec_adj_ladder(Ctx* c, double* taub, const double* lam2) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  conv_il(c, L->tau_b, lam2, d.po, d.pv, L->ld_pv);  // forward's tau_b as scratch
+  return bf16_nt(c, L->tau_b, L->ld_pv, L->vvvv_b, L->ld_pv, d.po, d.pv, d.pv, taub, d.pv, 1.0, 1.0);
+}
+
+int lowprec_adj_wmbej(Ctx* c, double* Tringb, const double* WTb) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  conv_il(c, L->WTbar_b, WTb, d.ov, d.ov, L->ld_ov);
+  return bf16_nt(c, L->WTbar_b, L->ld_ov, L->Doovv_b, L->ld_ov, d.ov, d.ov, d.ov, Tringb, d.ov, 0.5, 1.0);
+}
+
+int lowprec_adj_w2(Ctx* c, double* taub, const double* W2b) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  conv_il(c, L->W2bar_b, W2b, d.po, d.ov, L->ld_ov);
+  return bf16_nt(c, L->W2bar_b, L->ld_ov, L->ovvvT_b, L->ld_ov, d.po, d.pv, d.ov, taub, d.pv, 1.0, 1.0);
 }
 
 GemmDesc<float> f32_desc(const GemmDesc<double>& g, const float* A32, const float* B32) {

@@ -33,6 +33,11 @@ struct LowPrec {
   // ... and per-iteration amplitudes / intermediates
   float *t1_f = nullptr, *T2u_f = nullptr, *tauu_f = nullptr, *Tv_f = nullptr;
   float *W1_f = nullptr, *W3_f = nullptr, *W3u_f = nullptr, *W2_f = nullptr, *Fae_f = nullptr, *Fmi_f = nullptr;
+  // BF16 operands of the low-precision Lambda iterations: W_mbej^T and
+  // <ia||bc>^T (fixed during a backward) and the per-iteration W_mbej and
+  // W_mbij cotangents
+  uint16_t *WTt_b = nullptr, *ovvvT_b = nullptr, *WTbar_b = nullptr, *W2bar_b = nullptr;
+  int have_adj = 0, have_ovvvT = 0;
   std::vector<void*> allocs;
 };
 
@@ -50,6 +55,20 @@ int lowprec_w2(Ctx* c, int precision);
 int lowprec_wmbej(Ctx* c, int precision);
 int lowprec_ladder(Ctx* c, int precision);
 int lowprec_ring(Ctx* c, int precision);
+
+// Low-precision (BF16x3) pieces of the adjoint's reverse pass, used by the
+// Lambda iterations before their FP64 sweeps.  lowprec_adj_setup converts the
+// operands fixed at the converged amplitudes (call after the FP64 forward
+// intermediates are in place); the others add one O(N^6) product each:
+//   ring:   Tringb += Xb WT,  WTb += Xb Tring   (Xb = ring unpack of lam2)
+//   ladder: taub += lam2 <ab||cd>
+//   wmbej:  Tringb += 1/2 WTb Doovv
+//   w2:     taub += W2b <mb||ef>
+int lowprec_adj_setup(Ctx* c);
+int lowprec_adj_ring(Ctx* c, double* Tringb, double* WTb, const double* lam2);
+int lowprec_adj_ladder(Ctx* c, double* taub, const double* lam2);
+int lowprec_adj_wmbej(Ctx* c, double* Tringb, const double* WTb);
+int lowprec_adj_w2(Ctx* c, double* taub, const double* W2b);
 
 // FP32 GEMM with the operands A32 / B32 (strides of g unless overridden by the
 // caller in a copy of g) writing g's FP64 destination.

@@ -97,8 +97,9 @@ typedef struct ccsd_result {
 typedef struct ccsd_grad_info {
   int lambda_iterations;
   int converged;
-  double lambda_residual;  /* max |Lambda_n - Lambda_{n-1}| at exit */
+  double lambda_residual;     /* max |Lambda_n - Lambda_{n-1}| at exit */
   double seconds;
+  int lambda_lowp_iterations; /* of lambda_iterations, run with BF16x3 reverse products */
 } ccsd_grad_info;
 
 /* Fill *o with defaults. */
@@ -141,7 +142,10 @@ int ccsd_energy_trace(ccsd_ctx* c, double* out, int cap);
 /* Adjoint of the converged energy: solves the CCSD Lambda equations and
  * contracts the unrelaxed densities into dE/df (N*N, symmetrised) and
  * dE/d<pq||rs> in the packed block layout of ccsd_problem (symmetrised over
- * duplicate entries of oooo/ovov/vvvv).  Requires a converged solve. */
+ * duplicate entries of oooo/ovov/vvvv).  Requires a converged solve.  With
+ * precision BF16X3 the Lambda iterations follow the forward's mixed schedule
+ * (BF16x3 reverse products, then at least final_fp64_sweeps FP64 iterations);
+ * the gradient contraction is FP64. */
 int ccsd_backward(ccsd_ctx* c, double* dE_dfock, double* dE_doooo, double* dE_dooov, double* dE_doovv,
                   double* dE_dovov, double* dE_dovvv, double* dE_dvvvv, ccsd_grad_info* info);
 
@@ -151,6 +155,12 @@ int ccsd_backward(ccsd_ctx* c, double* dE_dfock, double* dE_doooo, double* dE_do
  * the amplitudes are kept (warm start for the next solve).  The caller keeps
  * symmetric blocks symmetric. */
 int ccsd_update_integrals(ccsd_ctx* c, int block, long n, const long* index, const double* delta);
+
+/* Page-locked host memory (hipHostMalloc) for large outputs such as the
+ * ccsd_backward gradients: copies into it run at DMA rate and overlap device
+ * work.  No counterpart in the reference (its gradients do not exist). */
+int ccsd_host_alloc(size_t bytes, void** out);
+int ccsd_host_free(void* p);
 
 /* Device bytes the context holds (or would hold for O, V before create). */
 size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision);

@@ -132,3 +132,24 @@ def test_backward_finite_differences_o20_v180():
     for kind, fd, pred in rows:
         assert abs(fd - pred) <= max(2e-5 * abs(fd), 1e-9), (kind, fd, pred)
     assert np.median(rel) <= 1e-6
+
+
+def test_backward_mixed_precision_matches_fp64():
+    """BF16x3 forward and Lambda iterations (each ending with FP64 sweeps) give
+    the FP64 gradient: relative error of the whole gradient below 1e-6."""
+    capi = _capi()
+    O, V, seed = 10, 40, 35
+    sc = 1.0 / np.sqrt(O * synth.p2(V))
+    out = {}
+    for prec in ("fp64", "bf16x3"):
+        with capi.Context.synthetic(O, V, seed, sc, precision=prec, diis_dim=8) as ctx:
+            r = ctx.solve(1e-10, 200)
+            g, info = ctx.backward()
+        out[prec] = (r, g, info)
+    (r64, g64, i64), (rmx, gmx, imx) = out["fp64"], out["bf16x3"]
+    assert i64["lambda_lowp_iterations"] == 0 and imx["lambda_lowp_iterations"] > 0
+    assert imx["converged"] == 1 and imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 3
+    num = np.sqrt(sum(float(np.sum((gmx[k] - g64[k]) ** 2)) for k in KEYS))
+    den = np.sqrt(sum(float(np.sum(g64[k] ** 2)) for k in KEYS))
+    assert num / den <= 1e-6, num / den
+    assert abs(rmx["e_corr"] - r64["e_corr"]) <= 1e-8
