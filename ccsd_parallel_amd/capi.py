@@ -297,9 +297,12 @@ def strerror(code):
     return load().ccsd_strerror(code).decode()
 
 
-def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision="fp64", tile=0):
+def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision="fp64", tile=0,
+                  elementwise=False):
     """C = alpha op(A) op(B) + beta C through the device GEMM engine; tile > 0
-    forces that tile configuration (index + 1) instead of the automatic choice."""
+    forces that tile configuration (index + 1) instead of the automatic choice;
+    elementwise forces single-element global loads where the FP64 engine would
+    load element pairs."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     Cm = np.ascontiguousarray(C_in, dtype=np.float64).copy()
@@ -307,7 +310,8 @@ def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision
     K = A.shape[0] if ta else A.shape[1]
     N = B.shape[0] if tb else B.shape[1]
     _check(load().ccsd_selftest_gemm(M, N, K, int(ta), int(tb), alpha, _ptr(A), _ptr(B), beta, _ptr(Cm),
-                                     PRECISIONS.get(precision, precision) | (int(tile) << 8)),
+                                     PRECISIONS.get(precision, precision) | (int(tile) << 8)
+                                     | (int(bool(elementwise)) << 16)),
            "ccsd_selftest_gemm")
     return Cm
 

@@ -625,7 +625,7 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
   double *dA = nullptr, *dB = nullptr, *dC = nullptr;
   void* ws = nullptr;
   int rc = CCSD_OK;
-  const size_t wsb = 64u << 20;
+  const size_t wsb = kGemmWorkspaceBytes;
   if (hipMalloc(&dA, (size_t)M * K * 8) != hipSuccess || hipMalloc(&dB, (size_t)K * N * 8) != hipSuccess ||
       hipMalloc(&dC, (size_t)M * N * 8) != hipSuccess || hipMalloc(&ws, wsb) != hipSuccess) {
     rc = CCSD_ERR_HIP;
@@ -642,7 +642,8 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
       g.sCm = N; g.sCn = 1;
       g.alpha = alpha; g.beta = beta;
       const int cfg = ((precision >> 8) & 0xff) - 1;  // forced tile configuration, -1 = automatic
-      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, -1) != hipSuccess) rc = CCSD_ERR_HIP;
+      const int pairs = (precision >> 16) & 1 ? 0 : -1;  // bit 16: element-wise loads only
+      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, 0, pairs) != hipSuccess) rc = CCSD_ERR_HIP;
     } else {
       // low-precision path wants A as [M][K] and B as [N][K]
       std::vector<double> Ak((size_t)M * K), Bk((size_t)N * K);

@@ -56,7 +56,7 @@ size_t plan_bytes(int O, int V, int precision, int diis_m) {
   // temporary ovov at setup shares nothing: count it
   n += d.ov * d.ov;
   size_t bytes = (size_t)n * sizeof(double);
-  bytes += 64ull << 20;  // GEMM workspace
+  bytes += kGemmWorkspaceBytes;  // GEMM workspace
   bytes += 1ull << 20;   // reductions
   bytes += lowprec_plan_bytes(d, precision);
   return bytes;
@@ -137,9 +137,9 @@ int plan_and_allocate(Ctx* c, int precision) {
   TRY(dalloc(c, &c->partial, 1 << 16));
   TRY(dalloc(c, &c->scal, 64));
   double* ws = nullptr;
-  TRY(dalloc(c, &ws, (64l << 20) / 8));
+  TRY(dalloc(c, &ws, (long)(kGemmWorkspaceBytes / 8)));
   c->ws.ptr = ws;
-  c->ws.bytes = 64ull << 20;
+  c->ws.bytes = kGemmWorkspaceBytes;
   // Low-precision operand copies are made on first use (lowprec_amplitudes),
   // after the integrals are in HBM; plan_bytes above already reserved them.
   for (auto& e : c->ev) HIPTRY(hipEventCreate(&e));

@@ -8,6 +8,11 @@
 //    contiguous 16-element run per k; double-buffered, one barrier per k-tile.
 //  * Global loads are coalesced along whichever operand stride is unit
 //    (template switches AK / BNM select the loop order); any strides work.
+//    Where the strides allow, each thread loads element pairs with one
+//    16-byte load (VEC); rows/columns past M/N are clamped, not masked, and
+//    only the ragged last k-tile is bounds-checked.
+//  * 4-wave tiles are compiled for two or more resident workgroups per CU, so
+//    one workgroup's barrier wait is covered by another's MFMAs.
 //  * Workgroup ids are remapped so that the tiles sharing one B column block
 //    run on the same XCD (the dispatcher deals workgroups round-robin over the
 //    8 XCDs, each with its own L2).
@@ -66,17 +71,17 @@ struct KArgs {
   int splits, kchunk;
 };
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool AK, bool BNM>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool AK, bool BNM, int WPE, bool VEC>
+__global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM / 16;
   constexpr int TN = BN / WN / 16;
-  static_assert(TM >= 1 && TN >= 1, "wave tile too small");
-  // LDS layouts.  A k-contiguous operand stays k-contiguous ([m][LDK]): the
-  // 16-byte stores of a row are contiguous and, with LDK = BK + 2 doubles, the
-  // 16 rows of one MFMA operand fetch fall in distinct banks.  An m/n-contiguous
-  // operand is stored [k][LDM]; the pitch BM + 16 puts consecutive k-rows of a
-  // fetch in opposite bank halves.
+  static_assert(TM >= 1 && TN >= 1 && TM * WM * 16 == BM && TN * WN * 16 == BN, "wave tile");
+  // LDS layouts.  A k-contiguous operand stays k-contiguous ([m][LDK]): with
+  // LDK = BK + 2 doubles the 16 rows of one MFMA operand fetch fall in distinct
+  // banks.  An m/n-contiguous operand is stored [k][LDM]; the pitch BM + 16
+  // puts consecutive k-rows of a fetch in opposite bank halves.  Every global
+  // load brings a pair of elements that are adjacent in LDS too.
   constexpr int LDK = BK + (sizeof(T) == 8 ? 2 : 4);
   constexpr int LDMA = BM + (sizeof(T) == 8 ? 16 : 32);
   constexpr int LDMB = BN + (sizeof(T) == 8 ? 16 : 32);
@@ -84,8 +89,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
   constexpr int BSZ = BNM ? BK * LDMB : BN * LDK;
   __shared__ __attribute__((aligned(16))) T As[2][ASZ];
   __shared__ __attribute__((aligned(16))) T Bs[2][BSZ];
-  auto aidx = [](int m, int k) { return AK ? m * LDK + k : k * LDMA + m; };
-  auto bidx = [](int k, int n) { return BNM ? k * LDMB + n : n * LDK + k; };
+  using pair_t = T __attribute__((ext_vector_type(2)));
 
   const GemmDesc<T>& d = p.d;
   // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8, so
@@ -106,6 +110,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
   const int m0 = tm * BM, n0 = tn * BN;
   const int kbeg = split * p.kchunk;
   const int kend = min(d.K, kbeg + p.kchunk);
+  const int M = d.M, N = d.N;
+  const long sAm = d.sAm, sAk = d.sAk, sBk = d.sBk, sBn = d.sBn;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -117,74 +123,89 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
   constexpr int BU = BK * BN / 2;
   constexpr int AIT = (AU + NT - 1) / NT;
   constexpr int BIT = (BU + NT - 1) / NT;
-  T ra[AIT][2], rb[BIT][2];
+  pair_t ra[AIT], rb[BIT];
 
-  auto load_tiles = [&](int k0) {
+  // Unit u of a tile -> (row, k) of its element pair.  A pair runs along the
+  // operand's contiguous index: k for AK / !BNM, m for !AK, n for BNM.
+  auto a_unit = [](int u, int& m, int& kk) {
+    if (AK) { m = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
+    else    { kk = u / (BM / 2); m = (u % (BM / 2)) * 2; }
+  };
+  auto b_unit = [](int u, int& n, int& kk) {
+    if (BNM) { kk = u / (BN / 2); n = (u % (BN / 2)) * 2; }
+    else     { n = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
+  };
+  // Rows past M and columns past N only feed outputs that the epilogue drops,
+  // so their loads are clamped into range rather than masked (a mask would
+  // have to wait for the load).  Only k is checked, and only in the ragged
+  // last tile: full tiles (`checked` false) load without any test.
+  // VEC: one 2-element load per pair (the host guarantees unit stride along
+  // the pair, even pitches and an aligned base; M or N even when the pair
+  // runs along m or n).
+  auto load_tiles = [&](int k0, bool checked) {
 #pragma unroll
     for (int it = 0; it < AIT; ++it) {
-      int u = it * NT + tid;
+      const int u = it * NT + tid;
       int m, kk;
-      if (AK) { m = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
-      else    { kk = u / (BM / 2); m = (u % (BM / 2)) * 2; }
-      int gm = m0 + m, gk = k0 + kk;
-      T v0 = 0, v1 = 0;
-      if (u < AU) {
-        if (AK) {
-          if (gm < d.M) {
-            if (gk < kend) v0 = A[gm * d.sAm + gk * d.sAk];
-            if (gk + 1 < kend) v1 = A[gm * d.sAm + (gk + 1) * d.sAk];
-          }
-        } else {
-          if (gk < kend) {
-            if (gm < d.M) v0 = A[gm * d.sAm + gk * d.sAk];
-            if (gm + 1 < d.M) v1 = A[(gm + 1) * d.sAm + gk * d.sAk];
-          }
+      a_unit(u, m, kk);
+      const int gm = m0 + m, gk = k0 + kk;
+      pair_t v = {T(0), T(0)};
+      if (AU % NT == 0 || u < AU) {
+        if (VEC && !checked) {
+          const T* src = AK ? A + (long)min(gm, M - 1) * sAm + gk : A + (long)gk * sAk + min(gm, M - 2);
+          v = *reinterpret_cast<const pair_t*>(src);
+        } else if (AK) {
+          const T* row = A + (long)min(gm, M - 1) * sAm;
+          if (!checked || gk < kend) v[0] = row[gk * sAk];
+          if (!checked || gk + 1 < kend) v[1] = row[(gk + 1) * sAk];
+        } else if (!checked || gk < kend) {
+          const T* kr = A + (long)gk * sAk;
+          v[0] = kr[min(gm, M - 1) * sAm];
+          v[1] = kr[min(gm + 1, M - 1) * sAm];
         }
       }
-      ra[it][0] = v0; ra[it][1] = v1;
+      ra[it] = v;
     }
 #pragma unroll
     for (int it = 0; it < BIT; ++it) {
-      int u = it * NT + tid;
+      const int u = it * NT + tid;
       int n, kk;
-      if (BNM) { kk = u / (BN / 2); n = (u % (BN / 2)) * 2; }
-      else     { n = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
-      int gn = n0 + n, gk = k0 + kk;
-      T v0 = 0, v1 = 0;
-      if (u < BU) {
-        if (BNM) {
-          if (gk < kend) {
-            if (gn < d.N) v0 = B[gk * d.sBk + gn * d.sBn];
-            if (gn + 1 < d.N) v1 = B[gk * d.sBk + (gn + 1) * d.sBn];
-          }
-        } else {
-          if (gn < d.N) {
-            if (gk < kend) v0 = B[gk * d.sBk + gn * d.sBn];
-            if (gk + 1 < kend) v1 = B[(gk + 1) * d.sBk + gn * d.sBn];
-          }
+      b_unit(u, n, kk);
+      const int gn = n0 + n, gk = k0 + kk;
+      pair_t v = {T(0), T(0)};
+      if (BU % NT == 0 || u < BU) {
+        if (VEC && !checked) {
+          const T* src = BNM ? B + (long)gk * sBk + min(gn, N - 2) : B + (long)min(gn, N - 1) * sBn + gk;
+          v = *reinterpret_cast<const pair_t*>(src);
+        } else if (!BNM) {
+          const T* col = B + (long)min(gn, N - 1) * sBn;
+          if (!checked || gk < kend) v[0] = col[gk * sBk];
+          if (!checked || gk + 1 < kend) v[1] = col[(gk + 1) * sBk];
+        } else if (!checked || gk < kend) {
+          const T* kr = B + (long)gk * sBk;
+          v[0] = kr[min(gn, N - 1) * sBn];
+          v[1] = kr[min(gn + 1, N - 1) * sBn];
         }
       }
-      rb[it][0] = v0; rb[it][1] = v1;
+      rb[it] = v;
     }
   };
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int it = 0; it < AIT; ++it) {
-      int u = it * NT + tid;
-      if (u >= AU) break;
-      if (AK) { int m = u / (BK / 2), kk = (u % (BK / 2)) * 2;
-                As[buf][aidx(m, kk)] = ra[it][0]; As[buf][aidx(m, kk + 1)] = ra[it][1]; }
-      else    { int kk = u / (BM / 2), m = (u % (BM / 2)) * 2;
-                As[buf][aidx(m, kk)] = ra[it][0]; As[buf][aidx(m + 1, kk)] = ra[it][1]; }
+      const int u = it * NT + tid;
+      if (AU % NT != 0 && u >= AU) break;
+      int m, kk;
+      a_unit(u, m, kk);
+      *reinterpret_cast<pair_t*>(&As[buf][AK ? m * LDK + kk : kk * LDMA + m]) = ra[it];
     }
 #pragma unroll
     for (int it = 0; it < BIT; ++it) {
-      int u = it * NT + tid;
-      if (u >= BU) break;
-      if (BNM) { int kk = u / (BN / 2), n = (u % (BN / 2)) * 2;
-                 Bs[buf][bidx(kk, n)] = rb[it][0]; Bs[buf][bidx(kk, n + 1)] = rb[it][1]; }
-      else     { int n = u / (BK / 2), kk = (u % (BK / 2)) * 2;
-                 Bs[buf][bidx(kk, n)] = rb[it][0]; Bs[buf][bidx(kk + 1, n)] = rb[it][1]; }
+      const int u = it * NT + tid;
+      if (BU % NT != 0 && u >= BU) break;
+      int n, kk;
+      b_unit(u, n, kk);
+      *reinterpret_cast<pair_t*>(&Bs[buf][BNM ? kk * LDMB + n : n * LDK + kk]) = rb[it];
     }
   };
 
@@ -195,31 +216,51 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs<T> p) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
-  if (ntiles > 0) {
-    load_tiles(kbeg);
-    store_tiles(0);
-    __syncthreads();
-  }
-  for (int kt = 0; kt < ntiles; ++kt) {
-    const int cur = kt & 1;
-    const bool more = (kt + 1) < ntiles;
-    if (more) load_tiles(kbeg + (kt + 1) * BK);
+  auto compute = [&](int cur) {
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
       T a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[cur][aidx(wm0 + i * 16 + (lane & 15), kr)];
+      for (int i = 0; i < TM; ++i) {
+        const int m = wm0 + i * 16 + (lane & 15);
+        a[i] = As[cur][AK ? m * LDK + kr : kr * LDMA + m];
+      }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[cur][bidx(kr, wn0 + j * 16 + (lane & 15))];
+      for (int j = 0; j < TN; ++j) {
+        const int n = wn0 + j * 16 + (lane & 15);
+        b[j] = Bs[cur][BNM ? kr * LDMB + n : n * LDK + kr];
+      }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
     }
+  };
+
+  // Full k-tiles run in the double-buffered loop (next tile's loads in flight
+  // during this tile's MFMAs, one barrier per tile); the ragged tail, if any,
+  // follows as one checked tile.  Keeping the checked loads out of the loop
+  // keeps each loaded pair in its own registers until its LDS store.
+  const int nfull = (kend > kbeg) ? (kend - kbeg) / BK : 0;
+  if (nfull > 0) {
+    load_tiles(kbeg, false);
+    store_tiles(0);
+    __syncthreads();
+  }
+  for (int kt = 0; kt < nfull; ++kt) {
+    const int cur = kt & 1;
+    const bool more = (kt + 1) < nfull;
+    if (more) load_tiles(kbeg + (kt + 1) * BK, false);
+    compute(cur);
     if (more) store_tiles(cur ^ 1);
     __syncthreads();
+  }
+  if (kbeg + nfull * BK < kend) {
+    load_tiles(kbeg + nfull * BK, true);
+    store_tiles(0);
+    __syncthreads();
+    compute(0);
   }
 
   // Epilogue (output row of each accumulator element from the MFMA trait).
@@ -321,60 +362,95 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
   if (lane == 0) write_out(d, (long)row * d.sCm, s);
 }
 
-// rate: FP64 TF/s of the config on a 12000^3 GEMM on MI355X (tile-sweep probe);
-// slots: resident workgroups per CU (LDS / register limited).
-struct Cfg { int BM, BN, BK, WM, WN; float rate; int slots; };
+// One tile configuration: BM x BN x BK on WM x WN waves.  slots: workgroups
+// resident per CU (LDS and register limited; the kernel is compiled for at
+// least `slots` waves per SIMD when it has 4 waves, so that two or more
+// workgroups overlap each other's barriers).  rate: FP64 TF/s with every slot
+// busy and paired loads (12000^3 GEMM on MI355X, tile-sweep probe).
+struct Cfg { int BM, BN, BK, WM, WN, slots; float rate; };
 constexpr Cfg kCfgs[] = {
-    {128, 256, 16, 2, 4, 54.4f, 1},
-    {256, 128, 16, 4, 2, 54.0f, 1},
-    {128, 128, 16, 2, 2, 39.1f, 2},
-    {64, 128, 16, 2, 2, 45.8f, 2},
-    {128, 64, 16, 2, 2, 45.6f, 2},
-    {64, 64, 16, 2, 2, 44.7f, 4},
-    {32, 64, 16, 1, 2, 25.0f, 4},
-    {64, 32, 16, 2, 1, 25.0f, 4},
+    {128, 256, 16, 2, 4, 1, 62.6f},
+    {256, 128, 16, 4, 2, 1, 62.2f},
+    {128, 128, 16, 2, 2, 2, 63.0f},
+    {64, 128, 16, 2, 2, 2, 58.8f},
+    {128, 64, 16, 2, 2, 2, 57.3f},
+    {64, 64, 16, 2, 2, 4, 54.2f},
+    {32, 64, 16, 1, 2, 4, 42.3f},
+    {64, 32, 16, 2, 1, 4, 38.7f},
     // 48-wide tiles for the O = 40 edges (M or N of 40 fills 83% instead of 62%)
-    {128, 48, 16, 4, 1, 42.0f, 2},
-    {48, 128, 16, 1, 4, 42.0f, 2},
-    {48, 64, 16, 1, 2, 38.0f, 4},
-    {64, 48, 16, 2, 1, 38.0f, 4},
+    {128, 48, 16, 4, 1, 2, 51.3f},
+    {48, 128, 16, 1, 4, 2, 53.3f},
+    {48, 64, 16, 1, 2, 4, 47.8f},
+    {64, 48, 16, 2, 1, 4, 45.8f},
+    // 112 = 7 x 16 rows: P2(40) = 780 fills 99.5% (128-row tiles: 87%)
+    {112, 128, 16, 1, 4, 2, 56.3f},
+    {128, 112, 16, 4, 1, 2, 62.0f},
 };
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <typename T, int BM, int BN, int BK, int WM, int WN>
-hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStream_t s) {
+template <typename T, int BM, int BN, int BK, int WM, int WN, int WPE, bool VEC>
+hipError_t launch_layout(const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStream_t s) {
   constexpr int NT = 64 * WM * WN;
-  if (AK && BNM)        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true>), grid, dim3(NT), 0, s, ka);
-  else if (AK && !BNM)  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false>), grid, dim3(NT), 0, s, ka);
-  else if (!AK && BNM)  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true>), grid, dim3(NT), 0, s, ka);
-  else                  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false>), grid, dim3(NT), 0, s, ka);
+  if (AK && BNM)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+  else if (AK && !BNM)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+  else if (!AK && BNM)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false, WPE, VEC>), grid, dim3(NT), 0, s, ka);
   return hipGetLastError();
 }
 
+template <typename T, int C>
+hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, bool vec, dim3 grid, hipStream_t s) {
+  constexpr Cfg g = kCfgs[C];
+  // waves-per-SIMD floor handed to the compiler: 4-wave tiles with 2+ slots
+  // must fit 512/slots registers
+  constexpr int wpe = (g.WM * g.WN == 4) ? g.slots : 1;
+  if (vec) return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, true>(ka, AK, BNM, grid, s);
+  return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, false>(ka, AK, BNM, grid, s);
+}
+
 template <typename T>
-hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStream_t s) {
+hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, bool vec, dim3 grid, hipStream_t s) {
   switch (c) {
-    case 0: return launch_cfg<T, 128, 256, 16, 2, 4>(ka, AK, BNM, grid, s);
-    case 1: return launch_cfg<T, 256, 128, 16, 4, 2>(ka, AK, BNM, grid, s);
-    case 2: return launch_cfg<T, 128, 128, 16, 2, 2>(ka, AK, BNM, grid, s);
-    case 3: return launch_cfg<T, 64, 128, 16, 2, 2>(ka, AK, BNM, grid, s);
-    case 4: return launch_cfg<T, 128, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
-    case 5: return launch_cfg<T, 64, 64, 16, 2, 2>(ka, AK, BNM, grid, s);
-    case 6: return launch_cfg<T, 32, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
-    case 7: return launch_cfg<T, 64, 32, 16, 2, 1>(ka, AK, BNM, grid, s);
-    case 8: return launch_cfg<T, 128, 48, 16, 4, 1>(ka, AK, BNM, grid, s);
-    case 9: return launch_cfg<T, 48, 128, 16, 1, 4>(ka, AK, BNM, grid, s);
-    case 10: return launch_cfg<T, 48, 64, 16, 1, 2>(ka, AK, BNM, grid, s);
-    default: return launch_cfg<T, 64, 48, 16, 2, 1>(ka, AK, BNM, grid, s);
+    case 0: return launch_cfg<T, 0>(ka, AK, BNM, vec, grid, s);
+    case 1: return launch_cfg<T, 1>(ka, AK, BNM, vec, grid, s);
+    case 2: return launch_cfg<T, 2>(ka, AK, BNM, vec, grid, s);
+    case 3: return launch_cfg<T, 3>(ka, AK, BNM, vec, grid, s);
+    case 4: return launch_cfg<T, 4>(ka, AK, BNM, vec, grid, s);
+    case 5: return launch_cfg<T, 5>(ka, AK, BNM, vec, grid, s);
+    case 6: return launch_cfg<T, 6>(ka, AK, BNM, vec, grid, s);
+    case 7: return launch_cfg<T, 7>(ka, AK, BNM, vec, grid, s);
+    case 8: return launch_cfg<T, 8>(ka, AK, BNM, vec, grid, s);
+    case 9: return launch_cfg<T, 9>(ka, AK, BNM, vec, grid, s);
+    case 10: return launch_cfg<T, 10>(ka, AK, BNM, vec, grid, s);
+    case 11: return launch_cfg<T, 11>(ka, AK, BNM, vec, grid, s);
+    case 12: return launch_cfg<T, 12>(ka, AK, BNM, vec, grid, s);
+    default: return launch_cfg<T, 13>(ka, AK, BNM, vec, grid, s);
   }
+}
+static_assert(kNumCfgs == 14, "launch_any covers every configuration");
+
+// Paired (2-element vector) loads are legal when each operand has unit stride
+// along its pair index, every other stride and batch stride is even, the base
+// is aligned to a pair, and a pair along m or n cannot straddle M or N.
+template <typename T>
+bool pairs_ok(const GemmDesc<T>& d, bool AK, bool BNM) {
+  auto even = [](long x) { return (x & 1) == 0; };
+  auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0; };
+  const bool a = AK ? (d.sAk == 1 && even(d.sAm)) : (d.sAm == 1 && even(d.sAk) && even(d.M));
+  const bool b = BNM ? (d.sBn == 1 && even(d.sBk) && even(d.N)) : (d.sBk == 1 && even(d.sBn));
+  return a && b && even(d.sA1) && even(d.sA2) && even(d.sB1) && even(d.sB2) && aligned(d.A) && aligned(d.B);
 }
 
 }  // namespace
 
 template <typename T>
 hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t stream,
-                GemmLaunchInfo* info, int force_config, int force_split) {
+                GemmLaunchInfo* info, int force_config, int force_split, int force_pairs) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
   const int nbatch = d.nb1 * d.nb2;
   if (nbatch <= 0 || nbatch > 65535) return hipErrorInvalidValue;
@@ -392,67 +468,59 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     const int rows_per_block = 4;
     hipLaunchKernelGGL(gemv_kernel<T>, dim3((d.M + rows_per_block - 1) / rows_per_block), dim3(64 * rows_per_block),
                        0, stream, d);
-    if (info) *info = GemmLaunchInfo{d.M, 1, 1, 1, -1};
+    if (info) *info = GemmLaunchInfo{d.M, 1, 1, 1, -1, 0};
     return hipGetLastError();
   }
   // A is "k-contiguous" unless its m stride is the unit one (and likewise B).
   const bool AK = !(d.sAm == 1 && d.sAk != 1);
   const bool BNM = !(d.sBk == 1 && d.sBn != 1);
+  const bool vec = force_pairs != 0 && pairs_ok(d, AK, BNM);
 
-  // Pick the tile shape with the least modelled time: padding waste times
-  // wave quantisation over all CUs, divided by the shape's measured rate.
-  int best = 0;
+  // Tile shape and split-K count with the least modelled time: each of the
+  // tiles x splits work units costs its padded flops at the configuration's
+  // rate, units run in rounds of (CUs x slots), and a split adds one pass over
+  // the partial slices.  Splits need at least 8 k-tiles each and room for
+  // their partials in the workspace.
+  const bool need_partial_for_batch = d.reduce_batch && nbatch > 1;
+  const size_t part_bytes_per = (size_t)d.M * d.N * nbatch * sizeof(T);
+  const long ws_slices = (long)(ws.bytes / std::max<size_t>(1, part_bytes_per));
+  if (need_partial_for_batch && ws_slices < 1) return hipErrorOutOfMemory;
+  const double flops = 2.0 * d.M * (double)d.N * d.K * nbatch;
+  int best = 0, kchunk = d.K;
   double best_t = 1e300;
   for (int c = 0; c < kNumCfgs; ++c) {
+    if (force_config >= 0 && force_config < kNumCfgs && c != force_config) continue;
     const Cfg& g = kCfgs[c];
-    long tm = (d.M + g.BM - 1) / g.BM, tn = (d.N + g.BN - 1) / g.BN;
-    double pad = (double(tm * g.BM) * double(tn * g.BN)) / (double(d.M) * d.N);
-    double tiles = double(tm * tn) * nbatch;
-    double slots = double(kNumCus) * g.slots;
-    double rounds = std::ceil(tiles / slots);
-    double quant = rounds * slots / tiles;
-    // few tiles and a long K are fixed by split-K below; do not penalise them twice
-    if (tiles < slots && d.K >= 16 * g.BK * 8) quant = 1.0;
-    double t = pad * quant / g.rate;
-    if (t < best_t) { best_t = t; best = c; }
+    const long tm = (d.M + g.BM - 1) / g.BM, tn = (d.N + g.BN - 1) / g.BN;
+    const double pad = (double(tm * g.BM) * double(tn * g.BN)) / (double(d.M) * d.N);
+    const long tiles = tm * tn * nbatch;
+    const double slots = double(kNumCus) * g.slots;
+    const long ktiles = (d.K + g.BK - 1) / g.BK;
+    long s_lo = 1, s_hi = 1;
+    if (force_split > 0) {
+      s_lo = s_hi = force_split;
+    } else if (ws_slices >= 2) {
+      s_hi = std::max(1L, std::min({ktiles / 8, ws_slices, 64L}));
+    }
+    for (long s = s_lo; s <= s_hi; ++s) {
+      int kc = (int)((d.K + s - 1) / s);
+      kc = ((kc + g.BK - 1) / g.BK) * g.BK;
+      const long sr = (d.K + kc - 1) / kc;
+      if (sr > 1 && sr > ws_slices) continue;
+      const double units = double(tiles) * sr;
+      const double rounds = std::ceil(units / slots);
+      double t = rounds * slots / units * pad * flops / (g.rate * 1e12);
+      if (sr > 1 || need_partial_for_batch) t += (sr + 2) * double(part_bytes_per) / 4e12 + 5e-6;
+      if (t < best_t) { best_t = t; best = c; kchunk = kc; }
+    }
   }
-  if (force_config >= 0 && force_config < kNumCfgs) best = force_config;
   const Cfg& g = kCfgs[best];
   const int tiles_m = (d.M + g.BM - 1) / g.BM;
   const int tiles_n = (d.N + g.BN - 1) / g.BN;
   const long tiles = (long)tiles_m * tiles_n;
-
-  // split-K when the output tiles cannot fill the chip and K is long
-  int splits = 1;
-  const bool need_partial_for_batch = d.reduce_batch && nbatch > 1;
-  if (force_split > 0) {
-    splits = force_split;
-  } else {
-    long work = tiles * nbatch;
-    const long target = 2L * kNumCus * g.slots;
-    if (work < target) {
-      long want = (target + work - 1) / work;
-      long kmax = std::max(1, d.K / (g.BK * 8));
-      splits = (int)std::max(1L, std::min(want, kmax));
-    }
-  }
-  const size_t part_bytes_per = (size_t)d.M * d.N * nbatch * sizeof(T);
-  bool use_partial = splits > 1 || need_partial_for_batch;
-  if (use_partial) {
-    size_t maxs = ws.bytes / std::max<size_t>(1, part_bytes_per);
-    if (maxs < 1) {
-      if (need_partial_for_batch) return hipErrorOutOfMemory;
-      splits = 1;
-      use_partial = false;
-    } else {
-      splits = (int)std::min<size_t>(splits, maxs);
-      splits = std::min(splits, 65535);
-    }
-  }
-  int kchunk = (d.K + splits - 1) / splits;
-  kchunk = ((kchunk + g.BK - 1) / g.BK) * g.BK;
-  splits = (d.K + kchunk - 1) / kchunk;
-  use_partial = splits > 1 || need_partial_for_batch;
+  const int splits = (d.K + kchunk - 1) / kchunk;
+  const bool use_partial = splits > 1 || need_partial_for_batch;
+  if (use_partial && (long)splits > ws_slices) return hipErrorOutOfMemory;
 
   KArgs<T> ka;
   ka.d = d;
@@ -464,7 +532,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   ka.kchunk = kchunk;
   if (ka.tiles_pad <= 0 || splits > 65535) return hipErrorInvalidValue;
   dim3 grid(ka.tiles_pad, nbatch, splits);
-  hipError_t e = launch_any<T>(best, ka, AK, BNM, grid, stream);
+  hipError_t e = launch_any<T>(best, ka, AK, BNM, vec, grid, stream);
   if (e != hipSuccess) return e;
   if (use_partial) {
     const long MN = (long)d.M * d.N;
@@ -480,11 +548,12 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     }
     e = hipGetLastError();
   }
-  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best};
+  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best, vec ? 1 : 0};
   return e;
 }
 
-template hipError_t gemm<double>(const GemmDesc<double>&, const GemmWorkspace&, hipStream_t, GemmLaunchInfo*, int, int);
-template hipError_t gemm<float>(const GemmDesc<float>&, const GemmWorkspace&, hipStream_t, GemmLaunchInfo*, int, int);
+template hipError_t gemm<double>(const GemmDesc<double>&, const GemmWorkspace&, hipStream_t, GemmLaunchInfo*, int, int,
+                                 int);
+template hipError_t gemm<float>(const GemmDesc<float>&, const GemmWorkspace&, hipStream_t, GemmLaunchInfo*, int, int, int);
 
 }  // namespace ccsd

@@ -42,11 +42,18 @@ struct GemmWorkspace {
 // Launch statistics of the last call (for tests and the profiler notes).
 struct GemmLaunchInfo {
   int tiles_m = 0, tiles_n = 0, batches = 0, splits = 0, config = 0;
+  int pairs = 0;  // 1 when the operands were loaded as element pairs
 };
 
+// Workspace the engine's callers allocate for split-K / batch-reduce partials.
+constexpr size_t kGemmWorkspaceBytes = 256ull << 20;
+
+// force_config: tile configuration index (-1 = modelled choice); force_split:
+// K splits (0 = modelled); force_pairs: 0 = element-wise loads even where
+// paired loads are legal (-1 = automatic).
 template <typename T>
 hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t stream,
-                GemmLaunchInfo* info = nullptr, int force_config = -1, int force_split = 0);
+                GemmLaunchInfo* info = nullptr, int force_config = -1, int force_split = 0, int force_pairs = -1);
 
 // Plain FP64 GEMV-like helper is just gemm with N=1.
 

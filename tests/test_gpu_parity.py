@@ -49,21 +49,26 @@ def test_gemm_fp64(M, N, K, ta, tb):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
-N_FP64_TILES = 12  # configurations of the generic MFMA engine
+N_FP64_TILES = 14  # configurations of the generic MFMA engine
 N_BF16_TILES = 4   # configurations of the BF16 hi/lo kernel
 
 
+@pytest.mark.parametrize("elementwise", [False, True])
 @pytest.mark.parametrize("tile", range(1, N_FP64_TILES + 1))
-@pytest.mark.parametrize("M,N,K,ta,tb", [(40, 40, 30000, 0, 1), (97, 301, 77, 1, 0), (300, 40, 300, 0, 0)])
-def test_gemm_fp64_every_tile(tile, M, N, K, ta, tb):
-    """Every tile shape, ragged edges, and the few-tile / long-K split path."""
+@pytest.mark.parametrize("M,N,K,ta,tb", [(40, 40, 30000, 0, 1), (97, 301, 77, 1, 0), (300, 40, 300, 0, 0),
+                                         (98, 302, 77, 1, 0), (66, 34, 61, 1, 1)])
+def test_gemm_fp64_every_tile(tile, M, N, K, ta, tb, elementwise):
+    """Every tile shape, ragged edges (M, N and the k tail), the few-tile /
+    long-K split path, and both load paths: element pairs (even pitches, all
+    four operand layouts among these shapes) and single elements."""
     capi = _ctx()
     rng = np.random.default_rng(tile * 100 + M)
     A = rng.standard_normal((K, M) if ta else (M, K))
     B = rng.standard_normal((N, K) if tb else (K, N))
     C0 = rng.standard_normal((M, N))
     ref = 1.3 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.4 * C0
-    out = capi.selftest_gemm(A, B, C0, 1.3, 0.4, ta=bool(ta), tb=bool(tb), precision="fp64", tile=tile)
+    out = capi.selftest_gemm(A, B, C0, 1.3, 0.4, ta=bool(ta), tb=bool(tb), precision="fp64", tile=tile,
+                             elementwise=elementwise)
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))) * np.sqrt(K / 1000 + 1)
 
 
