@@ -15,7 +15,7 @@
 //  and streamed to the host (opt.recompute_v4).
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
-//    until max|dLambda| < switch_factor * tol, then at least
+//    until max|dLambda| < lambda_switch_factor * tol, then at least
 //    final_fp64_sweeps FP64 iterations -- the forward's schedule, with the
 //    DIIS subspace kept across the switch.  The gradient pass is always FP64.
 // The reference has no adjoint; its forward algebra is restated in engine.hip.
@@ -673,7 +673,8 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
       // refine in FP64 from here on; the DIIS subspace is kept (at O=40/V=300:
       // 15 + 5 iterations against 15 + 7 with a reset), since the tight
       // max|dLambda| < 1e-10 test on FP64 updates pins the FP64 fixed point
-      if (delta < std::max(c->opt.switch_factor * tol, tol)) lowp = false;
+      const double f = c->opt.lambda_switch_factor > 0 ? c->opt.lambda_switch_factor : c->opt.switch_factor;
+      if (delta < std::max(f * tol, tol)) lowp = false;
       continue;
     }
     if (delta < tol && fp64_after >= c->opt.final_fp64_sweeps) { conv = true; break; }

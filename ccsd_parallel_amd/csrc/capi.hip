@@ -252,6 +252,7 @@ int ccsd_default_options(ccsd_options* o) {
   // Switch at |dE| < 100 tol: at O=40/V=300 (tol 1e-7) that is 7 BF16x3
   // iterations and the 3 FP64 sweeps, against 10 + 3 when switching at tol.
   o->switch_factor = 100.0;
+  o->lambda_switch_factor = 0.0;
   return CCSD_OK;
 }
 

@@ -372,7 +372,7 @@ constexpr Cfg kCfgs[] = {
     {128, 256, 16, 2, 4, 1, 62.6f},
     {256, 128, 16, 4, 2, 1, 62.2f},
     {128, 128, 16, 2, 2, 2, 63.0f},
-    {64, 128, 16, 2, 2, 2, 58.8f},
+    {64, 128, 16, 2, 2, 2, 56.5f},
     {128, 64, 16, 2, 2, 2, 57.3f},
     {64, 64, 16, 2, 2, 4, 54.2f},
     {32, 64, 16, 1, 2, 4, 42.3f},
@@ -479,13 +479,19 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   // Tile shape and split-K count with the least modelled time: each of the
   // tiles x splits work units costs its padded flops at the configuration's
   // rate, units run in rounds of (CUs x slots), and a split adds one pass over
-  // the partial slices.  Splits need at least 8 k-tiles each and room for
-  // their partials in the workspace.
+  // the partial slices.  The time is at least the HBM time of reading A once
+  // per column tile and B once per row tile (two at most: neighbouring tiles
+  // share panels in L2) plus the C traffic, which is what bounds skinny
+  // products such as M = N = O with K = O V^2.  Splits need at least 8
+  // k-tiles each and room for their partials in the workspace.
   const bool need_partial_for_batch = d.reduce_batch && nbatch > 1;
   const size_t part_bytes_per = (size_t)d.M * d.N * nbatch * sizeof(T);
   const long ws_slices = (long)(ws.bytes / std::max<size_t>(1, part_bytes_per));
   if (need_partial_for_batch && ws_slices < 1) return hipErrorOutOfMemory;
   const double flops = 2.0 * d.M * (double)d.N * d.K * nbatch;
+  const double a_bytes = double(d.M) * d.K * nbatch * sizeof(T), b_bytes = double(d.K) * d.N * nbatch * sizeof(T);
+  const double c_bytes = double(part_bytes_per) * (d.beta != T(0) ? 2 : 1);
+  constexpr double kHbm = 6e12;  // sustained HBM bytes/s
   int best = 0, kchunk = d.K;
   double best_t = 1e300;
   for (int c = 0; c < kNumCfgs; ++c) {
@@ -496,20 +502,25 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     const long tiles = tm * tn * nbatch;
     const double slots = double(kNumCus) * g.slots;
     const long ktiles = (d.K + g.BK - 1) / g.BK;
+    const double t_hbm = (a_bytes * std::min(tn, 2L) + b_bytes * std::min(tm, 2L) + c_bytes) / kHbm;
     long s_lo = 1, s_hi = 1;
     if (force_split > 0) {
       s_lo = s_hi = force_split;
     } else if (ws_slices >= 2) {
-      s_hi = std::max(1L, std::min({ktiles / 8, ws_slices, 64L}));
+      s_hi = std::max(1L, std::min({ktiles / 8, ws_slices, 1024L}));
     }
-    for (long s = s_lo; s <= s_hi; ++s) {
+    long s_prev = -1;
+    // split counts 1..8, then in steps of ~1/8 (at most ~40 candidates)
+    for (long s = s_lo; s <= s_hi; s = (s < 8) ? s + 1 : s + s / 8) {
       int kc = (int)((d.K + s - 1) / s);
       kc = ((kc + g.BK - 1) / g.BK) * g.BK;
       const long sr = (d.K + kc - 1) / kc;
+      if (sr == s_prev) continue;  // same k-chunking as the previous count
+      s_prev = sr;
       if (sr > 1 && sr > ws_slices) continue;
       const double units = double(tiles) * sr;
       const double rounds = std::ceil(units / slots);
-      double t = rounds * slots / units * pad * flops / (g.rate * 1e12);
+      double t = std::max(rounds * slots / units * pad * flops / (g.rate * 1e12), t_hbm);
       if (sr > 1 || need_partial_for_batch) t += (sr + 2) * double(part_bytes_per) / 4e12 + 5e-6;
       if (t < best_t) { best_t = t; best = c; kchunk = kc; }
     }

@@ -66,6 +66,8 @@ typedef struct ccsd_options {
   int diis_reset_on_switch; /* mixed modes: clear the DIIS subspace at the switch to FP64 (default 1) */
   double switch_factor;     /* mixed modes: switch once |dE| < switch_factor * tol (default 100: the FP64
                                sweeps then take |dE| down the last two decades) */
+  double lambda_switch_factor; /* backward, BF16X3: switch the Lambda iterations to FP64 once
+                                  max|dLambda| < this * 1e-10 (0 = switch_factor) */
 } ccsd_options;
 
 typedef struct ccsd_problem {
