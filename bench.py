@@ -13,6 +13,8 @@ Workload: O=40 occupied / V=300 virtual spin orbitals, seed 40300, FP64.
     averaged over the timed steps);
   * mixed forward + adjoint against FP64 forward + adjoint (wall time, |dE|,
     gradient relative error);
+  * the RHF input path at the same O/V (n = O/2 + V/2 spatial orbitals): the
+    four AO->MO quarter-transform GEMMs and the spin-orbital packing in HBM;
   * the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
@@ -136,6 +138,50 @@ def combined_forward_adjoint(capi, O, V, seed, scale, tol=1e-7):
     return res
 
 
+def rhf_setup(capi, O, V, seed=170, cpu_budget_s=5.0):
+    """Context set-up from RHF AO data (ccsd_create_rhf) at n = O/2 + V/2
+    spatial orbitals: device times of the AO->MO transform (4 GEMMs, 2 n^5
+    flops each) and of the packing of the six spin-orbital blocks, the wall
+    time of the whole create (host-to-device copy of the n^4 integrals
+    included), and the oracle's quarter transform (numpy BLAS) on a bounded
+    slab of the same tensor, extrapolated by flop count."""
+    nocc, n = O // 2, O // 2 + V // 2
+    rng = np.random.default_rng(seed)
+    # 8-fold symmetric, positive semi-definite synthetic (pq|rs) = sum_k L[pq,k] L[rs,k], L[pq] = L[qp]
+    K = 8
+    L = rng.standard_normal((n, n, K)) * (0.5 / np.sqrt(K))
+    L = 0.5 * (L + L.transpose(1, 0, 2))
+    Lm = L.reshape(n * n, K)
+    eri = (Lm @ Lm.T).reshape(n, n, n, n)
+    eps = np.concatenate([np.sort(rng.uniform(-1.5, -0.3, nocc)), np.sort(rng.uniform(0.2, 3.0, n - nocc))])
+    Cm = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    t0 = time.perf_counter()
+    ctx = capi.Context.from_rhf(eps, eri, nocc, mo_coeff=Cm, diis_dim=8)
+    wall = time.perf_counter() - t0
+    tm = ctx.setup_timings()
+    ctx.close()
+    flops = 4 * 2.0 * n ** 5
+    # CPU: one quarter pass on row slabs of the (n^3 x n) view, as the oracle's numpy restatement would do it
+    flat = eri.reshape(n ** 3, n)
+    rows = min(n ** 3, 1 << 20)
+    reps, el = 0, 0.0
+    t0 = time.perf_counter()
+    while el < cpu_budget_s and reps < 1000:
+        _ = flat[:rows] @ Cm
+        reps += 1
+        el = time.perf_counter() - t0
+    cpu_rate = 2.0 * rows * n * n * reps / el
+    return {
+        "n_spatial": n, "n_occ": nocc, "O": O, "V": V, "eri_bytes": eri.nbytes,
+        "ao_to_mo_ms": tm["ao_to_mo"], "pack_ms": tm["pack"], "create_wall_s": wall,
+        "ao_to_mo_tflops": flops / (tm["ao_to_mo"] * 1e-3) / 1e12 if tm["ao_to_mo"] > 0 else None,
+        "ao_to_mo_flops": flops,
+        "cpu_baseline": {"value": flops / cpu_rate, "unit": "s (AO->MO transform, extrapolated)", "kind": "port",
+                         "sample": f"numpy BLAS quarter pass on a {rows} x {n} slab, {reps} reps in {el:.1f}s, "
+                                   f"{cpu_rate / 1e9:.1f} GFLOP/s scaled to {flops:.3e} flop"},
+    }
+
+
 def timed_steps(ctx, steps, warmup, precision):
     """Wall time per step and HIP-event kernel times averaged over the timed steps."""
     ctx.iterate(warmup, precision)
@@ -162,6 +208,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-rhf", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU; the multi-GPU sharding is a design note (DESIGN.md section 7)",
@@ -227,6 +274,8 @@ def main():
         out["time_to_converge_1e-7"] = conv
     if not args.no_adjoint:
         out["forward_adjoint_1e-7"] = combined_forward_adjoint(capi, O, V, seed, scale)
+    if not args.no_rhf:
+        out["rhf_setup"] = rhf_setup(capi, O, V)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(O, V, seed, scale)
     print(json.dumps(out))

@@ -24,7 +24,7 @@ EXPORTED = (
     "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
     "ccsd_selftest_gemm", "ccsd_update_integrals",
-    "ccsd_host_alloc", "ccsd_host_free",
+    "ccsd_host_alloc", "ccsd_host_free", "ccsd_create_rhf", "ccsd_ao_to_mo",
     # include/ccsd_rhf.h (host RHF input producer)
     "ccsd_basis_size", "ccsd_rhf", "ccsd_ao_one_electron",
 )
@@ -44,6 +44,11 @@ class Options(C.Structure):
 class Problem(C.Structure):
     _fields_ = [("n_occ", C.c_int), ("n_virt", C.c_int)] + [
         (n, C.POINTER(C.c_double)) for n in ("fock", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")]
+
+
+class RhfProblem(C.Structure):
+    _fields_ = [("n_basis", C.c_int), ("n_occ", C.c_int), ("n_frozen", C.c_int), ("eps", C.POINTER(C.c_double)),
+                ("C", C.POINTER(C.c_double)), ("eri", C.POINTER(C.c_double))]
 
 
 class Result(C.Structure):
@@ -88,6 +93,8 @@ def load():
     lib.ccsd_default_options.argtypes = [C.POINTER(Options)]
     lib.ccsd_create.argtypes = [C.POINTER(Problem), C.POINTER(Options), C.POINTER(vp)]
     lib.ccsd_create_synthetic.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double, C.POINTER(Options), C.POINTER(vp)]
+    lib.ccsd_create_rhf.argtypes = [C.POINTER(RhfProblem), C.POINTER(Options), C.POINTER(vp)]
+    lib.ccsd_ao_to_mo.argtypes = [C.c_int, dp, dp, dp, C.c_int]
     lib.ccsd_solve.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Result)]
     lib.ccsd_iterate.argtypes = [vp, C.c_int, C.c_int, dp]
     lib.ccsd_reset.argtypes = [vp]
@@ -167,6 +174,25 @@ class Context:
         return cls(h, O, V)
 
     @classmethod
+    def from_rhf(cls, eps, eri, n_occ, mo_coeff=None, n_frozen=0, **opts):
+        """Context from closed-shell data: orbital energies eps[n], spatial
+        integrals eri[n,n,n,n] ((pq|rs), AO basis with coefficients
+        mo_coeff[mu, p], or MO basis when mo_coeff is None); transform and
+        spin-orbital packing run on the device (ccsd_create_rhf)."""
+        lib = load()
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        eri = np.ascontiguousarray(eri, dtype=np.float64)
+        n = eps.shape[0]
+        if eri.shape != (n, n, n, n):
+            raise ValueError("eri must be n x n x n x n")
+        Cm = None if mo_coeff is None else np.ascontiguousarray(mo_coeff, dtype=np.float64)
+        p = RhfProblem(n, int(n_occ), int(n_frozen), _ptr(eps), _ptr(Cm) if Cm is not None else None, _ptr(eri))
+        o = default_options(**opts)
+        h = C.c_void_p()
+        _check(lib.ccsd_create_rhf(C.byref(p), C.byref(o), C.byref(h)), "ccsd_create_rhf")
+        return cls(h, 2 * (int(n_occ) - int(n_frozen)), 2 * (n - int(n_occ)))
+
+    @classmethod
     def synthetic(cls, O, V, seed, scale, **opts):
         lib = load()
         o = default_options(**opts)
@@ -243,6 +269,13 @@ class Context:
         self.lib.ccsd_last_timings(self.h, _ptr(ms), 5)
         return dict(zip(("pp_ladder", "ring", "wmbej_quad", "wmbij", "other"), ms.tolist()))
 
+    def setup_timings(self):
+        """Device times (ms) of a ccsd_create_rhf context's set-up: the AO->MO
+        transform and the spin-orbital packing."""
+        ms = np.zeros(7)
+        self.lib.ccsd_last_timings(self.h, _ptr(ms), 7)
+        return {"ao_to_mo": float(ms[5]), "pack": float(ms[6])}
+
     def debug(self, name):
         n = self.lib.ccsd_debug_get(self.h, name.encode(), None, 0)
         if n < 0:
@@ -314,6 +347,16 @@ def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision
                                      | (int(bool(elementwise)) << 16)),
            "ccsd_selftest_gemm")
     return Cm
+
+
+def ao_to_mo(Cmat, ao, device=0):
+    """(pq|rs) = sum C[mu,p] C[nu,q] C[la,r] C[si,s] (mu nu|la si) on the device."""
+    Cm = np.ascontiguousarray(Cmat, dtype=np.float64)
+    ao = np.ascontiguousarray(ao, dtype=np.float64)
+    n = Cm.shape[0]
+    mo = np.empty_like(ao)
+    _check(load().ccsd_ao_to_mo(n, _ptr(Cm), _ptr(ao), _ptr(mo), device), "ccsd_ao_to_mo")
+    return mo
 
 
 def _molecule(Z, R, angstrom):

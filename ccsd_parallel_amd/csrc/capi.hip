@@ -335,6 +335,72 @@ int ccsd_create_synthetic(int n_occ, int n_virt, uint64_t seed, double scale, co
   return CCSD_OK;
 }
 
+int ccsd_create_rhf(const ccsd_rhf_problem* p, const ccsd_options* o, ccsd_ctx** out) {
+  if (!p || !out || !p->eps || !p->eri) return CCSD_ERR_ARG;
+  *out = nullptr;
+  const int n = p->n_basis, nocc = p->n_occ, nf = p->n_frozen;
+  if (n < 2 || n > 1024 || nf < 0 || nocc - nf < 1 || n - nocc < 1) return CCSD_ERR_ARG;
+  const size_t n4 = (size_t)n * n * n * n;
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(p->eps[i])) return CCSD_ERR_NONFINITE;
+  if (p->C)
+    for (size_t i = 0; i < (size_t)n * n; ++i)
+      if (!std::isfinite(p->C[i])) return CCSD_ERR_NONFINITE;
+  for (size_t i = 0; i < n4; ++i)
+    if (!std::isfinite(p->eri[i])) return CCSD_ERR_NONFINITE;
+  ccsd_ctx* h = new (std::nothrow) ccsd_ctx();
+  if (!h) return CCSD_ERR_ARG;
+  Ctx* c = &h->c;
+  // spin orbital P: occupied spatial nf + P/2, virtual spatial nocc + (P-O)/2
+  const int O = 2 * (nocc - nf), V = 2 * (n - nocc);
+  const long N = O + V;
+  c->h_f.assign((size_t)N * N, 0.0);
+  for (long P = 0; P < N; ++P) c->h_f[P * N + P] = p->eps[P < O ? nf + P / 2 : nocc + (P - O) / 2];
+  c->d = make_dims(O, V);
+  int rc = check_fock(c);
+  if (rc == CCSD_OK) rc = init_device(c, o, O, V);
+  if (rc == CCSD_OK) rc = setup_from_rhf(c, n, nocc, nf, p->C, p->eri);
+  if (rc == CCSD_OK) rc = mp2_start(c);
+  if (rc != CCSD_OK) {
+    if (rc == CCSD_ERR_HIP && !c->last_error.empty()) fprintf(stderr, "ccsd_create_rhf: %s\n", c->last_error.c_str());
+    release(c);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CCSD_OK;
+}
+
+int ccsd_ao_to_mo(int n, const double* C, const double* ao_eri, double* mo_eri, int device) {
+  if (n < 1 || n > 1024 || !C || !ao_eri || !mo_eri) return CCSD_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCSD_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev || hipSetDevice(device) != hipSuccess) return CCSD_ERR_ARG;
+  const size_t n4 = (size_t)n * n * n * n;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return CCSD_ERR_HIP;
+  if (2 * n4 * 8 + (size_t)n * n * 8 + kGemmWorkspaceBytes > free_b) return CCSD_ERR_BUDGET;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CCSD_ERR_HIP;
+  double *dE = nullptr, *dT = nullptr, *dC = nullptr;
+  void* ws = nullptr;
+  int rc = CCSD_OK;
+  if (hipMalloc(&dE, n4 * 8) != hipSuccess || hipMalloc(&dT, n4 * 8) != hipSuccess ||
+      hipMalloc(&dC, (size_t)n * n * 8) != hipSuccess || hipMalloc(&ws, kGemmWorkspaceBytes) != hipSuccess)
+    rc = CCSD_ERR_BUDGET;
+  if (rc == CCSD_OK && (hipMemcpyAsync(dE, ao_eri, n4 * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+                        hipMemcpyAsync(dC, C, (size_t)n * n * 8, hipMemcpyHostToDevice, s) != hipSuccess))
+    rc = CCSD_ERR_HIP;
+  if (rc == CCSD_OK) rc = ao_to_mo_device(dE, dT, dC, n, GemmWorkspace{ws, kGemmWorkspaceBytes}, s);
+  if (rc == CCSD_OK && (hipMemcpyAsync(mo_eri, dE, n4 * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+    rc = CCSD_ERR_HIP;
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(dE); (void)hipFree(dT); (void)hipFree(dC); (void)hipFree(ws);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
 void ccsd_destroy(ccsd_ctx* h) {
   if (!h) return;
   release(&h->c);
@@ -506,7 +572,7 @@ int ccsd_energy_trace(ccsd_ctx* h, double* out, int cap) {
 
 int ccsd_last_timings(ccsd_ctx* h, double* ms, int cap) {
   if (!h || !ms) return CCSD_ERR_ARG;
-  int n = std::min(cap, 5);
+  int n = std::min(cap, 7);
   for (int i = 0; i < n; ++i) ms[i] = h->c.timings_ms[i];
   return n;
 }

@@ -184,6 +184,54 @@ int setup_from_host(Ctx* c, const ccsd_problem* p) {
   return finish_static(c, c->X);
 }
 
+int setup_from_rhf(Ctx* c, int n, int nocc, int nfrozen, const double* C, const double* eri) {
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  const size_t n4 = (size_t)n * n * n * n;
+  const size_t need = n4 * 8 * (C ? 2 : 1) + (size_t)n * n * 8;
+  size_t free_b = 0, total_b = 0;
+  HIPTRY(hipMemGetInfo(&free_b, &total_b));
+  if (need > free_b) return CCSD_ERR_BUDGET;
+  double *d_eri = nullptr, *d_tmp = nullptr, *d_C = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(d_eri);
+    (void)hipFree(d_tmp);
+    (void)hipFree(d_C);
+  };
+  int rc = CCSD_OK;
+  if (hipMalloc(&d_eri, n4 * 8) != hipSuccess || (C && hipMalloc(&d_tmp, n4 * 8) != hipSuccess) ||
+      (C && hipMalloc(&d_C, (size_t)n * n * 8) != hipSuccess)) {
+    cleanup();
+    return CCSD_ERR_BUDGET;
+  }
+  if (hipMemcpyAsync(d_eri, eri, n4 * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+      (C && hipMemcpyAsync(d_C, C, (size_t)n * n * 8, hipMemcpyHostToDevice, s) != hipSuccess))
+    rc = CCSD_ERR_HIP;
+  // HIP events on the stream: [5] transform, [6] spin-orbital packing
+  if (rc == CCSD_OK && hipEventRecord(c->ev[10], s) != hipSuccess) rc = CCSD_ERR_HIP;
+  if (rc == CCSD_OK && C) rc = ao_to_mo_device(d_eri, d_tmp, d_C, n, c->ws, s);
+  if (rc == CCSD_OK && hipEventRecord(c->ev[11], s) != hipSuccess) rc = CCSD_ERR_HIP;
+  if (rc == CCSD_OK) {
+    // ovov is staged in X (same ov x ov shape) and consumed into G
+    double* dst[6] = {c->oooo, c->ooov, c->oovv, c->X, c->ovvv, c->vvvv};
+    for (int b = 0; b < 6; ++b) k_rhf_block(dst[b], b, d, d_eri, n, nocc, nfrozen, s);
+    if (hipGetLastError() != hipSuccess) rc = CCSD_ERR_HIP;
+  }
+  if (rc == CCSD_OK && hipEventRecord(c->ev[9], s) != hipSuccess) rc = CCSD_ERR_HIP;
+  if (rc == CCSD_OK) {
+    float t_tr = 0.f, t_pack = 0.f;
+    if (hipEventSynchronize(c->ev[9]) != hipSuccess || hipEventElapsedTime(&t_tr, c->ev[10], c->ev[11]) != hipSuccess ||
+        hipEventElapsedTime(&t_pack, c->ev[11], c->ev[9]) != hipSuccess)
+      rc = CCSD_ERR_HIP;
+    c->timings_ms[5] = t_tr;
+    c->timings_ms[6] = t_pack;
+  }
+  cleanup();
+  if (rc != CCSD_OK) return rc;
+  return finish_static(c, c->X);
+}
+
 int setup_synthetic(Ctx* c, uint64_t seed, double scale) {
   const Dims& d = c->d;
   hipStream_t s = c->stream;

@@ -54,12 +54,18 @@ struct Ctx {
   bool solved = false;
   double e_mp2 = 0.0, e_last = 0.0;
   std::vector<double> energies;
-  double timings_ms[5] = {0, 0, 0, 0, 0};
+  double timings_ms[7] = {0, 0, 0, 0, 0, 0, 0};  // [5], [6]: RHF set-up (transform, packing)
   hipEvent_t ev[12] = {};
   std::string last_error;
 };
 
 int setup_from_host(Ctx* c, const ccsd_problem* p);
+// RHF input: spatial integrals (pq|rs) [n^4] (AO basis with coefficients C
+// [n][n], C[mu*n + p], or MO basis with C null) -> packed spin-orbital blocks
+// in HBM.  Temporary device buffers of 1-2 n^4 doubles live only during the call.
+int setup_from_rhf(Ctx* c, int n, int nocc, int nfrozen, const double* C, const double* eri);
+// Four quarter transforms in place: eri [n^4] (AO) -> (pq|rs) (MO); tmp [n^4] scratch.
+int ao_to_mo_device(double* eri, double* tmp, const double* C, int n, const GemmWorkspace& ws, hipStream_t s);
 int setup_synthetic(Ctx* c, uint64_t seed, double scale);
 int plan_and_allocate(Ctx* c, int precision);
 int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool time_it);

@@ -591,6 +591,40 @@ __global__ void synth_block_kernel(double* out, int rk, int ck, long r0, long r1
   }
 }
 
+// Packed <PQ||RS> block of RHF spatial integrals (pq|rs), chemists' notation,
+// mo[((p n + q) n + r) n + s].  Spin orbital P < O is spatial orbital
+// nf + P/2 with spin P%2 (the nf lowest occupied orbitals are frozen); virtual
+// P = O + A is spatial orbital nocc + A/2 with spin A%2
+// (ccsd_memory_optimized.cpp:483-485, Prepear_AOs(nr_freeze) :2698).
+// <PQ||RS> = <PQ|RS> - <PQ|SR> with <PQ|RS> = (pr|qs) when spin(P) = spin(R)
+// and spin(Q) = spin(S), else 0 -- the values the reference's
+// Fill_integ*_2D slices form on the fly (:3574-4779, e.g. :3728, :3870).
+__device__ __forceinline__ void so_spatial(int P, const Dims& d, int nocc, int nf, long& sp, int& spin) {
+  if (P < d.O) { sp = nf + P / 2; spin = P & 1; }
+  else { sp = nocc + (P - d.O) / 2; spin = (P - d.O) & 1; }
+}
+
+__global__ void rhf_block_kernel(double* out, int rk, int ck, long nrows, long ncols, Dims d, const double* __restrict__ mo,
+                                 long n, int nocc, int nf) {
+  const long total = nrows * ncols;
+  GRID_LOOP(idx, total) {
+    const long r = idx / ncols, c = idx % ncols;
+    int P, Q, R, S;
+    axis_orbitals(rk, r, d, P, Q);
+    axis_orbitals(ck, c, d, R, S);
+    long p, q, rr, s;
+    int sP, sQ, sR, sS;
+    so_spatial(P, d, nocc, nf, p, sP);
+    so_spatial(Q, d, nocc, nf, q, sQ);
+    so_spatial(R, d, nocc, nf, rr, sR);
+    so_spatial(S, d, nocc, nf, s, sS);
+    double v = 0.0;
+    if (sP == sR && sQ == sS) v += mo[((p * n + rr) * n + q) * n + s];  // <PQ|RS> = (pr|qs)
+    if (sP == sS && sQ == sR) v -= mo[((p * n + s) * n + q) * n + rr];  // <PQ|SR> = (ps|qr)
+    out[idx] = v;
+  }
+}
+
 }  // namespace
 
 // ============================================================================ launchers
@@ -728,6 +762,14 @@ void k_synth_block(double* out, int block, long r0, long r1, Dims d, uint64_t se
   long ncols = (ck[block] == 0) ? d.po : (ck[block] == 1 ? d.ov : d.pv);
   long n = (r1 - r0) * ncols;
   LAUNCH(synth_block_kernel, n, out, rk[block], ck[block], r0, r1, ncols, d, seed, scale);
+}
+
+void k_rhf_block(double* out, int block, Dims d, const double* mo, int n, int nocc, int nfrozen, hipStream_t s) {
+  static const int rk[6] = {0, 0, 0, 1, 1, 2};
+  static const int ck[6] = {0, 1, 2, 1, 2, 2};
+  auto extent = [&](int kind) { return kind == 0 ? d.po : (kind == 1 ? d.ov : d.pv); };
+  const long nr = extent(rk[block]), nc = extent(ck[block]);
+  LAUNCH(rhf_block_kernel, nr * nc, out, rk[block], ck[block], nr, nc, d, mo, (long)n, nocc, nfrozen);
 }
 
 }  // namespace ccsd

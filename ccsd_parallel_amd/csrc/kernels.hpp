@@ -108,5 +108,9 @@ void k_scatter_add(double* x, const long* idx, const double* delta, long n, int 
 // -- synthetic generator (counter-based; restates oracle/synth.py) ------------------------
 // block id: 0 oooo, 1 ooov, 2 oovv, 3 ovov, 4 ovvv, 5 vvvv.  rows [r0, r1) of the block.
 void k_synth_block(double* out, int block, long r0, long r1, Dims d, uint64_t seed, double scale, hipStream_t s);
+// Whole packed block `block` (0 oooo, 1 ooov, 2 oovv, 3 ovov, 4 ovvv, 5 vvvv) of
+// <PQ||RS> from RHF spatial MO integrals (pq|rs) [n^4] in HBM; nocc doubly
+// occupied spatial orbitals of which the lowest nfrozen are not correlated.
+void k_rhf_block(double* out, int block, Dims d, const double* mo, int n, int nocc, int nfrozen, hipStream_t s);
 
 }  // namespace ccsd

@@ -115,6 +115,34 @@ int ccsd_create(const ccsd_problem* p, const ccsd_options* o, ccsd_ctx** out);
 int ccsd_create_synthetic(int n_occ, int n_virt, uint64_t seed, double scale, const ccsd_options* o,
                           ccsd_ctx** out);
 
+/* Closed-shell (RHF) input in spatial orbitals, the data the reference's
+ * CCSD_Memory_optimized::CCSD pulls from its Hartree_Fock_Solver
+ * (ccsd_memory_optimized.cpp:49-74 and Return_Field_Q at :2987): orbital
+ * energies, MO coefficients and the AO two-electron integrals. */
+typedef struct ccsd_rhf_problem {
+  int n_basis;        /* n: basis functions = spatial MOs */
+  int n_occ;          /* doubly occupied spatial orbitals */
+  int n_frozen;       /* lowest occupied spatial orbitals left out of the correlation (frozen core) */
+  const double* eps;  /* canonical orbital energies [n] (the Fock matrix is diagonal) */
+  const double* C;    /* MO coefficients C[mu*n + p]; NULL when eri is already in the MO basis */
+  const double* eri;  /* (mu nu|la si), chemists' notation, [((mu*n + nu)*n + la)*n + si], n^4 */
+} ccsd_rhf_problem;
+
+/* Create a context from RHF data.  Replaces the reference's AO->MO step and
+ * integral slicing (Prepear_AOs, ccsd_memory_optimized.cpp:2698-3522;
+ * Fill_integ*_2D, :3574-4779): the integrals are copied to HBM, transformed
+ * to the MO basis by four MFMA GEMMs there (when C is given), and packed into
+ * the antisymmetrised spin-orbital blocks of ccsd_problem on the device
+ * (spin orbital P: spatial n_frozen + P/2 or n_occ + (P-O)/2, spin P%2;
+ * O = 2 (n_occ - n_frozen), V = 2 (n - n_occ)).  The transform needs 2 n^4
+ * doubles of HBM during the call (CCSD_ERR_BUDGET when they are not free). */
+int ccsd_create_rhf(const ccsd_rhf_problem* p, const ccsd_options* o, ccsd_ctx** out);
+
+/* The AO->MO transform alone, on the device: mo_eri (pq|rs) =
+ * sum C[mu][p] C[nu][q] C[la][r] C[si][s] (mu nu|la si), all n^4 arrays in the
+ * ccsd_rhf_problem layout (host buffers). */
+int ccsd_ao_to_mo(int n, const double* C, const double* ao_eri, double* mo_eri, int device);
+
 /* Iterate from the MP2 guess (or from amplitudes set by ccsd_set_amplitudes)
  * until |dE| < tol or max_iter updates.  Returns CCSD_ERR_NOT_CONVERGED (and a
  * filled result) when max_iter is reached. */
@@ -169,7 +197,9 @@ size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision);
 
 /* Per-kernel-class timing of the last iteration (milliseconds), HIP events on
  * the context's stream: [0] pp ladder, [1] ring, [2] W_mbej quadratic,
- * [3] W_mbij, [4] everything else.  Returns the count written. */
+ * [3] W_mbij, [4] everything else; for a context made by ccsd_create_rhf,
+ * [5] the AO->MO transform and [6] the spin-orbital packing at creation.
+ * Returns the count written (at most 7). */
 int ccsd_last_timings(ccsd_ctx* c, double* ms, int cap);
 
 void ccsd_destroy(ccsd_ctx* c);

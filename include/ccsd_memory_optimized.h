@@ -13,6 +13,9 @@
 //    ccsd_memory_optimized.cpp:49, 50, 68, 74, 2987).
 //  * A second constructor takes RHF MO data directly (orbital energies and
 //    spatial MO integrals (pq|rs)), for callers without an HF object.
+//  * The AO->MO transform and the spin-orbital integral blocks are built on
+//    the GPU (ccsd_create_rhf) instead of in Prepear_AOs / Fill_integ*_2D on
+//    the host.
 #ifndef CCSD_MEMORY_OPTIMIZED_DROPIN_H
 #define CCSD_MEMORY_OPTIMIZED_DROPIN_H
 
@@ -95,8 +98,7 @@ class CCSD_Memory_optimized {
   int mpi_size() const { return size; }
 
  private:
-  int build_spin_orbital_problem();
-  int mo_from_provider(double toler);
+  int gather_from_provider(double toler);
 
   int n_Nuclei = 0, n_Electrons = 0, rank = 0, size = 1;
   ccsd_compat::vec Z;
@@ -106,7 +108,9 @@ class CCSD_Memory_optimized {
   bool freeze_core = false;
 
   int n_ = 0, nocc_ = 0;
-  std::vector<double> eps_, mo_eri_;
+  // spatial integrals (pq|rs): AO basis with coefficients C_[mu*n + p] (HF
+  // provider), or MO basis with C_ empty (direct constructor)
+  std::vector<double> eps_, eri_, C_;
   double e_hf_ = 0.0, e_corr_ = 0.0;
   ccsd_options opt_{};
   bool have_opt_ = false;
@@ -114,8 +118,6 @@ class CCSD_Memory_optimized {
   bool restarted_ = false;
   ccsd_result result_{};
   int status_ = 0;
-  // spin-orbital packed blocks handed to the C ABI
-  std::vector<double> f_, oooo_, ooov_, oovv_, ovov_, ovvv_, vvvv_;
 };
 
 #endif

@@ -6,21 +6,27 @@ spin orbital P has spatial orbital P//2 and spin P%2 within the occupied and
 within the virtual range (:483-485); <PQ||RS> = <PQ|RS> - <PQ|SR> with
 <PQ|RS> = (pr|qs) when spins match pairwise (e.g. :3728, :3870).
 
+Frozen core (Prepear_AOs(nr_freeze), :2698): the lowest nfrozen occupied
+spatial orbitals are left out, so occupied spin orbital P is spatial orbital
+nfrozen + P//2.
+
 Pinned by: spin-adapted identities in the tests (closed-shell MP2 from the
 spatial formula sum (ia|jb)[2(ia|jb) - (ib|ja)]/D equals the spin-orbital MP2,
-and the CCSD energy of a 2-electron RHF system equals its FCI).
+and the CCSD energy of a 2-electron RHF system equals its FCI); the frozen-core
+problem equals the full problem of the integrals with the core rows and
+columns removed.
 """
 import numpy as np
 
 from . import synth
 
 
-def spin_orbital_tensor(eps, mo, nocc):
-    """Full <PQ||RS> and Fock over all spin orbitals (occupied first)."""
+def spin_orbital_tensor(eps, mo, nocc, nfrozen=0):
+    """Full <PQ||RS> and Fock over all correlated spin orbitals (occupied first)."""
     n = len(eps)
     nv = n - nocc
-    O, V = 2 * nocc, 2 * nv
-    spatial = np.array([P // 2 for P in range(O)] + [nocc + A // 2 for A in range(V)])
+    O, V = 2 * (nocc - nfrozen), 2 * nv
+    spatial = np.array([nfrozen + P // 2 for P in range(O)] + [nocc + A // 2 for A in range(V)])
     spin = np.array([P % 2 for P in range(O)] + [A % 2 for A in range(V)])
     ch = mo[np.ix_(spatial, spatial, spatial, spatial)]  # (pr|qs) needs reordering below
     # <PQ|RS> = (PR|QS) delta(sP,sR) delta(sQ,sS)
@@ -32,8 +38,8 @@ def spin_orbital_tensor(eps, mo, nocc):
     return f, g, O, V
 
 
-def problem_from_rhf(eps, mo, nocc):
-    f, g, O, V = spin_orbital_tensor(eps, mo, nocc)
+def problem_from_rhf(eps, mo, nocc, nfrozen=0):
+    f, g, O, V = spin_orbital_tensor(eps, mo, nocc, nfrozen)
     return dict(O=O, V=V, f=f, **synth.pack_dense(synth.blocks_from_full(g, O), O, V))
 
 

@@ -50,6 +50,24 @@ def test_rhf_two_electron_ccsd_is_fci():
     assert abs(E - fci2.fci_two_electron(f, g)[0]) < 1e-11
 
 
+def test_frozen_core_problem_is_the_problem_without_the_core():
+    """Freezing the lowest occupied orbital gives exactly the problem built from
+    the integrals with that orbital's rows and columns deleted."""
+    n, nocc = 8, 3
+    eps, mo = rhf_spin.random_rhf(n, nocc, 9)
+    frozen = rhf_spin.problem_from_rhf(eps, mo, nocc, nfrozen=1)
+    keep = np.arange(1, n)
+    sliced = rhf_spin.problem_from_rhf(eps[keep], mo[np.ix_(keep, keep, keep, keep)], nocc - 1)
+    assert frozen["O"] == sliced["O"] == 4 and frozen["V"] == sliced["V"]
+    for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv"):
+        assert np.array_equal(frozen[k], sliced[k]), k
+    # and it changes the energy (the core is not a no-op)
+    full = rhf_spin.problem_from_rhf(eps, mo, nocc)
+    e_fc = ccsd_ref.energy(ccsd_ref.Ints.from_problem(frozen), *ccsd_ref.mp2_guess(ccsd_ref.Ints.from_problem(frozen)))
+    e_all = ccsd_ref.energy(ccsd_ref.Ints.from_problem(full), *ccsd_ref.mp2_guess(ccsd_ref.Ints.from_problem(full)))
+    assert abs(e_fc - e_all) > 1e-6
+
+
 def _write_input(path, n, nocc, ehf, eps, C, ao, mo):
     with open(path, "wb") as fh:
         np.array([n, nocc], dtype=np.int32).tofile(fh)

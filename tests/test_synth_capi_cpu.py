@@ -120,6 +120,28 @@ def test_host_validation_errors():
     with pytest.raises(capi.CCSDError) as e:
         capi.Context.from_problem(deg)
     assert e.value.code == -6
+    # RHF input: shape/occupation checks and non-finite inputs are rejected on the host
+    eps = np.array([-1.0, -0.5, 0.5, 1.0])
+    eri = np.zeros((4, 4, 4, 4))
+    for n_occ, n_frozen in ((0, 0), (4, 0), (2, 2), (2, -1)):
+        with pytest.raises(capi.CCSDError) as e:
+            capi.Context.from_rhf(eps, eri, n_occ, n_frozen=n_frozen)
+        assert e.value.code == -1, (n_occ, n_frozen)
+    bad_eri = eri.copy()
+    bad_eri[1, 2, 3, 0] = np.inf
+    with pytest.raises(capi.CCSDError) as e:
+        capi.Context.from_rhf(eps, bad_eri, 2)
+    assert e.value.code == -4
+    with pytest.raises(capi.CCSDError) as e:
+        capi.Context.from_rhf(eps, eri, 2, mo_coeff=np.full((4, 4), np.nan))
+    assert e.value.code == -4
+    deg_eps = np.array([-1.0, 0.5, 0.5, 1.0])  # occupied 2 and virtual 3 coincide
+    with pytest.raises(capi.CCSDError) as e:
+        capi.Context.from_rhf(deg_eps, eri, 2)
+    assert e.value.code == -6
+    assert lib.ccsd_create_rhf(None, None, C.byref(h)) == -1
+    A = np.eye(3)
+    assert lib.ccsd_ao_to_mo(0, capi._ptr(A), capi._ptr(A), capi._ptr(A), 0) == -1
 
 
 def test_no_device_is_reported():
@@ -136,4 +158,10 @@ def test_no_device_is_reported():
     A = np.ones((2, 2))
     with pytest.raises(capi.CCSDError) as e:
         capi.selftest_gemm(A, A, A)
+    assert e.value.code == -7
+    with pytest.raises(capi.CCSDError) as e:
+        capi.ao_to_mo(np.eye(3), np.zeros((3, 3, 3, 3)))
+    assert e.value.code == -7
+    with pytest.raises(capi.CCSDError) as e:
+        capi.Context.from_rhf(np.array([-1.0, -0.5, 0.5, 1.0]), np.zeros((4, 4, 4, 4)), 2)
     assert e.value.code == -7
