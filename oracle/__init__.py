@@ -12,6 +12,10 @@ ccsd_ref     spin-orbital CCSD restating CCSD_Memory_optimized::CCSD's factorisa
 ccsd_sg      independent textbook Stanton-Gauss CCSD (cross-check of ccsd_ref)
 fci2         brute-force 2-electron FCI (CCSD is exact for two electrons)
 adjoint_ref  CCSD Lambda / gradient oracle via torch autograd on ccsd_ref (FP64, CPU)
+rhf_spin     RHF spatial integrals -> packed spin-orbital problem (frozen core included)
+ccsd_cpu.cpp the reference's loop shapes in C++/OpenMP (one dot product per output,
+             threads over output rows), built by cpu_build.py: the CPU baseline
+             bench.py times beside the GPU
 
 Every function states what pins it; see the module docstrings.
 """

@@ -157,3 +157,35 @@ def test_golden_fixtures():
         assert abs(E - case["e_corr"]) < 1e-10
         blk = synth.packed_block("vvvv", O, V, seed, scale, rows=(0, 1))
         assert abs(float(blk.sum()) - case["vvvv_row0_sum"]) < 1e-14
+
+
+def test_cpu_restatement_loops_compute_the_ladder_and_ring_terms():
+    """The C++ loops the CPU baseline times (oracle/ccsd_cpu.cpp) compute the
+    oracle's terms: packed dot products over c<d give the pp ladder
+    1/2 sum_cd <ab||cd> tau_ij^cd, and dot products over (m,e) give the ring
+    sum_me t_im^ae W_mbej, at random amplitudes."""
+    from oracle import cpu_build
+    try:
+        cpu_build.load()
+    except RuntimeError:
+        cpu_build.build(verbose=False)
+    O, V, seed = 4, 9, 5
+    prob = synth.make_problem(O, V, seed, scale=0.1)
+    I = ccsd_ref.Ints.from_problem(prob)
+    rng = np.random.default_rng(3)
+    t1 = 0.05 * rng.standard_normal((O, V))
+    t2 = 0.05 * rng.standard_normal((O, O, V, V))
+    t2 = t2 - t2.transpose(1, 0, 2, 3)
+    t2 = t2 - t2.transpose(0, 1, 3, 2)
+    _, _, _, terms2, X = ccsd_ref.rhs(I, t1, t2, parts=True)
+    tau_p = synth.pack_pairs(synth.pack_pairs(X["tau"], O, 0), V, 1)          # [i<j][c<d]
+    lad = cpu_build.dot_rows(prob["vvvv"], tau_p)                              # [a<b][i<j]
+    pp_ref = synth.pack_pairs(synth.pack_pairs(terms2["pp"], O, 0), V, 1)      # [i<j][a<b]
+    assert np.abs(lad.T - pp_ref).max() < 1e-14
+    assert np.abs(pp_ref).max() > 1e-4
+    # ring: rows (i,a) of t_im^ae, columns (j,b) of W_mbej, both over (m,e)
+    Tring = t2.transpose(0, 2, 1, 3).reshape(O * V, O * V)                      # [(i,a)][(m,e)]
+    Wcol = X["W4"].transpose(3, 1, 0, 2).reshape(O * V, O * V)                  # [(j,b)][(m,e)] = W[m,b,e,j]
+    ring = cpu_build.dot_rows(Tring, Wcol).reshape(O, V, O, V).transpose(0, 2, 1, 3)  # [i,j,a,b]
+    ref = np.einsum("imae,mbej->ijab", t2, X["W4"])
+    assert np.abs(ring - ref).max() < 1e-14 and np.abs(ref).max() > 1e-4
