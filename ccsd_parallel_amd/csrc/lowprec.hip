@@ -175,28 +175,34 @@ __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, f
 //
 // Block tile BM x BN over WM x WN waves (wave tile (BM/WM) x (BN/WN)), k-tiles
 // of 32.  Each thread moves 16-byte pieces of the interleaved rows (8 per row
-// per k-tile: 4 hi + 4 lo) through registers into double-buffered LDS, laid out
-// [hi|lo][row][k] with an 80-byte row pitch (16 lanes reading one k-slice hit
-// distinct banks).  The next k-tile's loads are issued unconditionally before
-// the current tile's MFMAs (the last iteration reloads its own tile into the
-// idle buffer) so they stay in flight across the compute; rows past M / N are
-// clamped to the last row (they only feed accumulator rows that are never
-// stored).  At 256 x 256 the LDS is 160 KB and a block holds one CU.
+// per k-tile: 4 hi + 4 lo) through registers into double-buffered LDS.  A row
+// keeps its global order (128 bytes: hi0 lo0 hi1 lo1 .. lo3) with the 16-byte
+// chunk index XOR-swizzled by (row >> 1) & 7, and no padding: 16 lanes store
+// two whole rows (256 contiguous bytes), and the 16 rows of one MFMA operand
+// fetch land in distinct banks.  The next k-tile's loads are issued
+// unconditionally before the current tile's MFMAs (the last iteration reloads
+// its own tile into the idle buffer) so they stay in flight across the
+// compute; rows past M / N are clamped to the last row (they only feed
+// accumulator rows that are never stored).  At 256 x 256 the LDS is 128 KB and
+// a block holds one CU; the 128 x 128 tile (64 KB, 4 waves) is compiled for two
+// resident blocks per CU.
 //
 // Tile order: XCD-aware (linear workgroup x runs on XCD x % 8, so XCD j takes a
 // contiguous range of tiles); with group > 0 the tiles of that range are
 // walked in groups of `group` tile rows so the 32 concurrent tiles of an XCD
 // share operand panels in its L2.
 template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
+__global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_nt_kernel(Bf16Args p) {
   constexpr int NT = 64 * WM * WN;
   constexpr int BK = 32;
-  constexpr int LDK = BK + 8;
+  constexpr int RP = 2 * BK;  // row pitch in bf16 elements: 128 bytes, hi and lo interleaved per 8
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
   constexpr int CA = BM * 8 / NT, CB = BN * 8 / NT;  // 16-byte pieces per thread per k-tile
   static_assert(CA * NT == BM * 8 && CB * NT == BN * 8, "tile/threads mismatch");
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * RP];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * RP];
+  // LDS slot of 16-byte chunk q of tile row m
+  auto slot = [](int m, int q) { return m * RP + ((q ^ ((m >> 1) & 7)) << 3); };
 
   const int gx = p.tiles_pad;
   const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
 
-  // piece u = it*NT + tid: row u/8, piece q = u%8 -> chunk q/2, hi (q even) or lo
+  // piece u = it*NT + tid: row u/8, chunk q = u%8 (k-slice q/2, hi for q even, lo for q odd)
   const uint16_t* pa[CA];
   const uint16_t* pb[CB];
   int sa[CA], sb[CB];
@@ -220,15 +226,14 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
   for (int it = 0; it < CA; ++it) {
     const int u = it * NT + tid, m = u >> 3, q = u & 7;
     pa[it] = p.A + (long)min(m0 + m, p.M - 1) * 2 * p.lda + q * 8;
-    sa[it] = (q & 1) * (BM * LDK) + m * LDK + (q >> 1) * 8;
+    sa[it] = slot(m, q);
   }
 #pragma unroll
   for (int it = 0; it < CB; ++it) {
     const int u = it * NT + tid, n = u >> 3, q = u & 7;
     pb[it] = p.B + (long)min(n0 + n, p.N - 1) * 2 * p.ldb + q * 8;
-    sb[it] = (q & 1) * (BN * LDK) + n * LDK + (q >> 1) * 8;
+    sb[it] = slot(n, q);
   }
-  uint4 ra[CA], rb[CB];
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -236,35 +241,37 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
   const int nk = p.Kp / BK;
-#pragma unroll
-  for (int it = 0; it < CA; ++it) ra[it] = *(const uint4*)(pa[it]);
-#pragma unroll
-  for (int it = 0; it < CB; ++it) rb[it] = *(const uint4*)(pb[it]);
-#pragma unroll
-  for (int it = 0; it < CA; ++it) *(uint4*)&sA[0][0][sa[it]] = ra[it];
-#pragma unroll
-  for (int it = 0; it < CB; ++it) *(uint4*)&sB[0][0][sb[it]] = rb[it];
-  __syncthreads();
-  const int kofs = 8 * (lane >> 4);
-  for (int kt = 0; kt < nk; ++kt) {
-    const long kn = 2L * min(kt + 1, nk - 1) * BK;
+  // global -> registers for k-tile t (clamped: the tail reloads the last tile into the idle buffer)
+  auto load = [&](int t, uint4 (&ra)[CA], uint4 (&rb)[CB]) {
+    const long kn = 2L * min(t, nk - 1) * BK;
 #pragma unroll
     for (int it = 0; it < CA; ++it) ra[it] = *(const uint4*)(pa[it] + kn);
 #pragma unroll
     for (int it = 0; it < CB; ++it) rb[it] = *(const uint4*)(pb[it] + kn);
-    const int buf = kt & 1;
+  };
+  auto store = [&](int buf, const uint4 (&ra)[CA], const uint4 (&rb)[CB]) {
+#pragma unroll
+    for (int it = 0; it < CA; ++it) *(uint4*)&sA[buf][sa[it]] = ra[it];
+#pragma unroll
+    for (int it = 0; it < CB; ++it) *(uint4*)&sB[buf][sb[it]] = rb[it];
+  };
+  // operand fetch: lane l reads tile row 16i + (l & 15) (a multiple of 16 plus
+  // l & 15, so its swizzle is ((l & 15) >> 1) & 7), k-slice l >> 4
+  const int swz = (lane >> 1) & 7, ks = lane >> 4;
+  const int oh = ((2 * ks) ^ swz) << 3, ol = ((2 * ks + 1) ^ swz) << 3;
+  auto compute = [&](int buf) {
     bf16x8_t ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-      const int r = (wm0 + i * 16 + (lane & 15)) * LDK + kofs;
-      ah[i] = *(const bf16x8_t*)&sA[buf][0][r];
-      al[i] = *(const bf16x8_t*)&sA[buf][1][r];
+      const int r = (wm0 + i * 16 + (lane & 15)) * RP;
+      ah[i] = *(const bf16x8_t*)&sA[buf][r + oh];
+      al[i] = *(const bf16x8_t*)&sA[buf][r + ol];
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      const int r = (wn0 + j * 16 + (lane & 15)) * LDK + kofs;
-      const bf16x8_t bh = *(const bf16x8_t*)&sB[buf][0][r];
-      const bf16x8_t bl = *(const bf16x8_t*)&sB[buf][1][r];
+      const int r = (wn0 + j * 16 + (lane & 15)) * RP;
+      const bf16x8_t bh = *(const bf16x8_t*)&sB[buf][r + oh];
+      const bf16x8_t bl = *(const bf16x8_t*)&sB[buf][r + ol];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
@@ -272,11 +279,18 @@ __global__ __launch_bounds__(64 * WM * WN) void bf16x3_nt_kernel(Bf16Args p) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
       }
     }
-    const int nb = buf ^ 1;
-#pragma unroll
-    for (int it = 0; it < CA; ++it) *(uint4*)&sA[nb][0][sa[it]] = ra[it];
-#pragma unroll
-    for (int it = 0; it < CB; ++it) *(uint4*)&sB[nb][0][sb[it]] = rb[it];
+  };
+
+  // one tile ahead: tile kt+1 in flight during tile kt's MFMAs (a second
+  // register stage does not fit the 256 registers of 2 waves per SIMD)
+  uint4 ra[CA], rb[CB];
+  load(0, ra, rb);
+  store(0, ra, rb);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    load(kt + 1, ra, rb);
+    compute(kt & 1);
+    store((kt & 1) ^ 1, ra, rb);
     __syncthreads();
   }
 #pragma unroll
