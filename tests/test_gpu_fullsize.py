@@ -1,0 +1,77 @@
+"""GPU parity at the workload sizes themselves (O=20/V=180 and O=40/V=300).
+
+The dense oracle cannot hold these tensors (V^4 doubles), so the checks use
+what stays exact at any size:
+  * the first energy: E(MP2 guess) = sum_{i<j,a<b} <ij||ab>^2 / D_ij^ab
+    (ccsd_memory_optimized.cpp:473-502), from the host generator's packed
+    <ij||ab> block and orbital energies;
+  * the packed pp + hh ladder (Fill_t2_new, :2093): for a set of columns a<b
+    at the edges and the middle of the P2(V) range,
+    Rp[:, ab] = <ij||ab> + sum_{c<d} tau[ij][cd] <cd||ab> + sum_{m<n} W_mnij[mn][ij] tau[mn][ab],
+    with <ab||cd> rows regenerated on the host for just those pairs and tau,
+    W_mnij taken from the engine's own buffers (their construction is pinned
+    against the oracle at small sizes in test_gpu_parity.py);
+  * the ring GEMM (Fill_W5, :6240): rows (i,a) of X = Tring W_mbej^T, from the
+    engine's Tring and W_mbej buffers.
+These exercise the production tile / split-K configurations at the real
+shapes (P2(O) = 190 / 780 rows, K = P2(V) = 16110 / 44850, OV = 3600 / 12000).
+"""
+import numpy as np
+import pytest
+
+from oracle import synth
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(20, 180, 20180), (40, 300, 40300)]
+
+
+def _capi():
+    from ccsd_parallel_amd import capi
+    return capi
+
+
+def _columns(n, width=48):
+    mid = n // 2
+    return np.unique(np.concatenate([np.arange(width), np.arange(mid, mid + width), np.arange(n - width, n)]))
+
+
+@pytest.mark.parametrize("O,V,seed", SHAPES)
+def test_full_size_products_and_first_energy(O, V, seed):
+    capi = _capi()
+    scale = 1.0 / np.sqrt(O * synth.p2(V))
+    po, pv, ov = synth.p2(O), synth.p2(V), O * V
+    print("case: O", O, "V", V, "seed", seed, flush=True)
+    with capi.Context.synthetic(O, V, seed, scale, diis_dim=8) as ctx:
+        # first energy = closed form on the generator's <ij||ab>
+        oovv = synth.packed_block("oovv", O, V, seed, scale)               # [i<j][a<b]
+        eo, ev = synth.orbital_energies(O, V, seed)
+        ip, iq = synth.pair_list(O)
+        ap, aq = synth.pair_list(V)
+        D = (eo[ip] + eo[iq])[:, None] - (ev[ap] + ev[aq])[None, :]
+        e_mp2 = float(np.sum(oovv * oovv / D))
+        t1m, t2m = ctx.get_amplitudes()                                     # the MP2 guess
+        assert abs(ctx.energy(t1m, t2m) - e_mp2) < 1e-12 * abs(e_mp2)
+        assert np.abs(t2m - oovv / D).max() < 1e-14 * np.abs(t2m).max()
+
+        # right-hand side at random amplitudes around the MP2 guess
+        rng = np.random.default_rng(seed + 1)
+        t1 = 1e-3 * rng.standard_normal((O, V))
+        t2 = t2m * (1.0 + 0.1 * rng.standard_normal(t2m.shape))
+        ctx.residual(t1, t2)
+        tau = ctx.debug("tau").reshape(po, pv)
+        W1 = ctx.debug("W1").reshape(po, po)
+        Rp = ctx.debug("Rp").reshape(po, pv)
+        cols = _columns(pv)
+        vv = np.concatenate([synth.packed_block("vvvv", O, V, seed, scale, rows=(int(c), int(c) + 1)) for c in cols])
+        ref = oovv[:, cols] + tau @ vv.T + W1.T @ tau[:, cols]
+        err = np.abs(Rp[:, cols] - ref).max()
+        assert err < 1e-12 * np.abs(ref).max(), err
+
+        Tring = ctx.debug("Tring").reshape(ov, ov)
+        WT = ctx.debug("WT").reshape(ov, ov)
+        X = ctx.debug("X").reshape(ov, ov)
+        rows = _columns(ov, 32)
+        ref = Tring[rows] @ WT.T
+        err = np.abs(X[rows] - ref).max()
+        assert err < 1e-12 * np.abs(ref).max(), err
