@@ -13,6 +13,7 @@ Workload: O=40 occupied / V=300 virtual spin orbitals, seed 40300, FP64.
     averaged over the timed steps);
   * mixed forward + adjoint against FP64 forward + adjoint (wall time, |dE|,
     gradient relative error);
+  * the plain FP64 configuration O=20 / V=180 (seed 20180) converged to 1e-10;
   * the RHF input path at the same O/V (n = O/2 + V/2 spatial orbitals): the
     four AO->MO quarter-transform GEMMs and the spin-orbital packing in HBM;
   * the CPU oracle timed on a bounded sample of the same workload.
@@ -293,6 +294,14 @@ def main():
             conv[p]["abs_dE_vs_fp64"] = abs(conv[p]["e_corr"] - conv["fp64"]["e_corr"])
             conv[p]["speedup_vs_fp64"] = conv["fp64"]["seconds_total"] / conv[p]["seconds_total"]
         out["time_to_converge_1e-7"] = conv
+    if not args.no_converge:
+        # plain FP64 configuration: O=20 / V=180, seed 20180, converged to 1e-10
+        o2, v2, s2 = 20, 180, 20180
+        c = capi.Context.synthetic(o2, v2, s2, 1.0 / np.sqrt(o2 * p2(v2)), precision="fp64", diis_dim=8)
+        r = c.solve(1e-10, 200, allow_unconverged=True)
+        c.close()
+        out["fp64_o20_v180_1e-10"] = {k: r[k] for k in ("e_corr", "e_mp2", "iterations", "converged", "seconds_total",
+                                                         "seconds_per_iter", "last_delta_e", "max_delta_t2")}
     if not args.no_adjoint:
         out["forward_adjoint_1e-7"] = combined_forward_adjoint(capi, O, V, seed, scale)
     if not args.no_rhf:

@@ -12,7 +12,10 @@ what stays exact at any size:
     W_mnij taken from the engine's own buffers (their construction is pinned
     against the oracle at small sizes in test_gpu_parity.py);
   * the ring GEMM (Fill_W5, :6240): rows (i,a) of X = Tring W_mbej^T, from the
-    engine's Tring and W_mbej buffers.
+    engine's Tring and W_mbej buffers;
+  * the converged FP64 solve at O=20/V=180 to 1e-10 (the plain FP64
+    configuration): its amplitudes are a fixed point, D t = R(t), to the
+    reported max|dt2|, and the returned energy is E(t*).
 These exercise the production tile / split-K configurations at the real
 shapes (P2(O) = 190 / 780 rows, K = P2(V) = 16110 / 44850, OV = 3600 / 12000).
 """
@@ -75,3 +78,25 @@ def test_full_size_products_and_first_energy(O, V, seed):
         ref = Tring[rows] @ WT.T
         err = np.abs(X[rows] - ref).max()
         assert err < 1e-12 * np.abs(ref).max(), err
+
+
+def test_fp64_solve_o20_v180_is_a_fixed_point():
+    capi = _capi()
+    O, V, seed = 20, 180, 20180
+    scale = 1.0 / np.sqrt(O * synth.p2(V))
+    eo, ev = synth.orbital_energies(O, V, seed)
+    ip, iq = synth.pair_list(O)
+    ap, aq = synth.pair_list(V)
+    D2 = (eo[ip] + eo[iq])[:, None] - (ev[ap] + ev[aq])[None, :]
+    D1 = eo[:, None] - ev[None, :]
+    with capi.Context.synthetic(O, V, seed, scale, diis_dim=8) as ctx:
+        r = ctx.solve(1e-10, 200)
+        assert r["converged"] and r["last_delta_e"] < 1e-10
+        t1, t2 = ctx.get_amplitudes()
+        assert abs(ctx.energy(t1, t2) - r["e_corr"]) < 1e-12
+        R1, R2 = ctx.residual(t1, t2)
+        d2 = np.abs(R2 / D2 - t2).max()
+        d1 = np.abs(R1 / D1 - t1).max()
+        print("E", r["e_corr"], "iterations", r["iterations"], "max|dt1|", d1, "max|dt2|", d2, flush=True)
+        assert d2 < 1e-8 and d1 < 1e-8
+        assert d2 <= 10 * max(r["max_delta_t2"], 1e-13)
