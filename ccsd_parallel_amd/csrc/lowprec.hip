@@ -242,18 +242,20 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
 
   const int nk = p.Kp / BK;
   // global -> registers for k-tile t (clamped: the tail reloads the last tile into the idle buffer)
-  auto load = [&](int t, uint4 (&ra)[CA], uint4 (&rb)[CB]) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 ra[CA], rb[CB];
+  auto load = [&](int t) {
     const long kn = 2L * min(t, nk - 1) * BK;
 #pragma unroll
-    for (int it = 0; it < CA; ++it) ra[it] = *(const uint4*)(pa[it] + kn);
+    for (int it = 0; it < CA; ++it) ra[it] = *(const u32x4*)(pa[it] + kn);
 #pragma unroll
-    for (int it = 0; it < CB; ++it) rb[it] = *(const uint4*)(pb[it] + kn);
+    for (int it = 0; it < CB; ++it) rb[it] = *(const u32x4*)(pb[it] + kn);
   };
-  auto store = [&](int buf, const uint4 (&ra)[CA], const uint4 (&rb)[CB]) {
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int it = 0; it < CA; ++it) *(uint4*)&sA[buf][sa[it]] = ra[it];
+    for (int it = 0; it < CA; ++it) *(u32x4*)&sA[buf][sa[it]] = ra[it];
 #pragma unroll
-    for (int it = 0; it < CB; ++it) *(uint4*)&sB[buf][sb[it]] = rb[it];
+    for (int it = 0; it < CB; ++it) *(u32x4*)&sB[buf][sb[it]] = rb[it];
   };
   // operand fetch: lane l reads tile row 16i + (l & 15) (a multiple of 16 plus
   // l & 15, so its swizzle is ((l & 15) >> 1) & 7), k-slice l >> 4
@@ -281,16 +283,23 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
     }
   };
 
-  // one tile ahead: tile kt+1 in flight during tile kt's MFMAs (a second
-  // register stage does not fit the 256 registers of 2 waves per SIMD)
-  uint4 ra[CA], rb[CB];
-  load(0, ra, rb);
-  store(0, ra, rb);
+  // Each iteration first stores tile kt+1 (loaded during the previous
+  // iteration) into the idle buffer, which the previous barrier released,
+  // then issues the loads of tile kt+2 and computes tile kt: the LDS stores
+  // overlap the MFMAs instead of sitting between the last MFMA and the
+  // barrier.  (A second register stage does not fit the 256 registers of 2
+  // waves per SIMD.)
+  load(0);
+  store(0);
+  load(1);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    load(kt + 1, ra, rb);
+    store((kt & 1) ^ 1);
+    // keep the loads below the stores: hoisted above them, the new tile would
+    // need a second register set while the old one still waits to be stored
+    __builtin_amdgcn_sched_barrier(0);
+    load(kt + 2);
     compute(kt & 1);
-    store((kt & 1) ^ 1, ra, rb);
     __syncthreads();
   }
 #pragma unroll
