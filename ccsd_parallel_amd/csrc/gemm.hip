@@ -238,23 +238,47 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
     }
   };
 
-  // Full k-tiles run in the double-buffered loop (next tile's loads in flight
-  // during this tile's MFMAs, one barrier per tile); the ragged tail, if any,
-  // follows as one checked tile.  Keeping the checked loads out of the loop
-  // keeps each loaded pair in its own registers until its LDS store.
+  // Full k-tiles run in the double-buffered loop (the next tiles' loads in
+  // flight during this tile's MFMAs, one barrier per tile); the ragged tail,
+  // if any, follows as one checked tile.  Keeping the checked loads out of the
+  // loop keeps each loaded pair in its own registers until its LDS store.
+  // Paired loads: each iteration first stores tile kt+1 (loaded during the
+  // previous iteration) into the idle buffer, released by the previous
+  // barrier, then loads tile kt+2 into the same registers and computes tile
+  // kt, so the LDS stores overlap the MFMAs.  Loads past the last full tile
+  // reload it (harmless: that buffer is not read again before the next
+  // barrier).  Element-wise loads hold twice the address registers and keep
+  // the load / compute / store order, which fits without spilling.
   const int nfull = (kend > kbeg) ? (kend - kbeg) / BK : 0;
-  if (nfull > 0) {
-    load_tiles(kbeg, false);
-    store_tiles(0);
-    __syncthreads();
-  }
-  for (int kt = 0; kt < nfull; ++kt) {
-    const int cur = kt & 1;
-    const bool more = (kt + 1) < nfull;
-    if (more) load_tiles(kbeg + (kt + 1) * BK, false);
-    compute(cur);
-    if (more) store_tiles(cur ^ 1);
-    __syncthreads();
+  if constexpr (VEC) {
+    if (nfull > 0) {
+      load_tiles(kbeg, false);
+      store_tiles(0);
+      load_tiles(kbeg + min(1, nfull - 1) * BK, false);
+      __syncthreads();
+    }
+    for (int kt = 0; kt < nfull; ++kt) {
+      store_tiles((kt & 1) ^ 1);
+      // keep the loads below the stores (one register set)
+      __builtin_amdgcn_sched_barrier(0);
+      load_tiles(kbeg + min(kt + 2, nfull - 1) * BK, false);
+      compute(kt & 1);
+      __syncthreads();
+    }
+  } else {
+    if (nfull > 0) {
+      load_tiles(kbeg, false);
+      store_tiles(0);
+      __syncthreads();
+    }
+    for (int kt = 0; kt < nfull; ++kt) {
+      const int cur = kt & 1;
+      const bool more = (kt + 1) < nfull;
+      if (more) load_tiles(kbeg + (kt + 1) * BK, false);
+      compute(cur);
+      if (more) store_tiles(cur ^ 1);
+      __syncthreads();
+    }
   }
   if (kbeg + nfull * BK < kend) {
     load_tiles(kbeg + nfull * BK, true);
@@ -366,25 +390,26 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
 // resident per CU (LDS and register limited; the kernel is compiled for at
 // least `slots` waves per SIMD when it has 4 waves, so that two or more
 // workgroups overlap each other's barriers).  rate: FP64 TF/s with every slot
-// busy and paired loads (12000^3 GEMM on MI355X, tile-sweep probe).
+// busy and paired loads (12000^3 GEMM on MI355X, tile-sweep probe, divided
+// by that shape's round fill as modelled below).
 struct Cfg { int BM, BN, BK, WM, WN, slots; float rate; };
 constexpr Cfg kCfgs[] = {
-    {128, 256, 16, 2, 4, 1, 62.6f},
-    {256, 128, 16, 4, 2, 1, 62.2f},
-    {128, 128, 16, 2, 2, 2, 63.0f},
-    {64, 128, 16, 2, 2, 2, 56.5f},
-    {128, 64, 16, 2, 2, 2, 57.3f},
-    {64, 64, 16, 2, 2, 4, 54.2f},
-    {32, 64, 16, 1, 2, 4, 42.3f},
-    {64, 32, 16, 2, 1, 4, 38.7f},
+    {128, 256, 16, 2, 4, 1, 59.8f},
+    {256, 128, 16, 4, 2, 1, 59.9f},
+    {128, 128, 16, 2, 2, 2, 66.2f},
+    {64, 128, 16, 2, 2, 2, 59.7f},
+    {128, 64, 16, 2, 2, 2, 58.0f},
+    {64, 64, 16, 2, 2, 4, 55.8f},
+    {32, 64, 16, 1, 2, 4, 42.7f},
+    {64, 32, 16, 2, 1, 4, 39.0f},
     // 48-wide tiles for the O = 40 edges (M or N of 40 fills 83% instead of 62%)
-    {128, 48, 16, 4, 1, 2, 51.3f},
-    {48, 128, 16, 1, 4, 2, 53.3f},
-    {48, 64, 16, 1, 2, 4, 47.8f},
-    {64, 48, 16, 2, 1, 4, 45.8f},
+    {128, 48, 16, 4, 1, 2, 52.7f},
+    {48, 128, 16, 1, 4, 2, 58.8f},
+    {48, 64, 16, 1, 2, 4, 45.5f},
+    {64, 48, 16, 2, 1, 4, 43.6f},
     // 112 = 7 x 16 rows: P2(40) = 780 fills 99.5% (128-row tiles: 87%)
-    {112, 128, 16, 1, 4, 2, 56.3f},
-    {128, 112, 16, 4, 1, 2, 62.0f},
+    {112, 128, 16, 1, 4, 2, 63.2f},
+    {128, 112, 16, 4, 1, 2, 63.0f},
 };
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
