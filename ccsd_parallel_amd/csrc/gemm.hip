@@ -216,25 +216,33 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
+  // One k-step (4 deep) of MFMA operands from LDS buffer `cur`, and its MFMAs.
+  struct Frag { T a[TM], b[TN]; };
+  auto frag = [&](int cur, int ks, Frag& f) {
+    const int kr = ks + (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int m = wm0 + i * 16 + (lane & 15);
+      f.a[i] = As[cur][AK ? m * LDK + kr : kr * LDMA + m];
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = wn0 + j * 16 + (lane & 15);
+      f.b[j] = Bs[cur][BNM ? kr * LDMB + n : n * LDK + kr];
+    }
+  };
+  auto mma = [&](const Frag& f) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(f.a[i], f.b[j], acc[i][j]);
+  };
   auto compute = [&](int cur) {
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
-      const int kr = ks + (lane >> 4);
-      T a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int m = wm0 + i * 16 + (lane & 15);
-        a[i] = As[cur][AK ? m * LDK + kr : kr * LDMA + m];
-      }
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int n = wn0 + j * 16 + (lane & 15);
-        b[j] = Bs[cur][BNM ? kr * LDMB + n : n * LDK + kr];
-      }
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+      Frag f;
+      frag(cur, ks, f);
+      mma(f);
     }
   };
 
@@ -249,22 +257,45 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   // reload it (harmless: that buffer is not read again before the next
   // barrier).  Element-wise loads hold twice the address registers and keep
   // the load / compute / store order, which fits without spilling.
+  // The MFMA operands are double-buffered in registers as well: the LDS reads
+  // of k-step s+1 are issued before the MFMAs of k-step s.  The tile's one
+  // barrier sits before its last k-step, after every read of buffer `cur`
+  // and every store of tile kt+1: behind it the first operands of tile kt+1
+  // are read while the last MFMAs of tile kt run.
   const int nfull = (kend > kbeg) ? (kend - kbeg) / BK : 0;
   if constexpr (VEC) {
+    constexpr int S = BK / 4;
+    static_assert(S % 2 == 0, "operand sets alternate per k-step");
+    Frag f[2];
     if (nfull > 0) {
       load_tiles(kbeg, false);
       store_tiles(0);
       load_tiles(kbeg + min(1, nfull - 1) * BK, false);
       __syncthreads();
+      frag(0, 0, f[0]);
     }
     for (int kt = 0; kt < nfull; ++kt) {
-      store_tiles((kt & 1) ^ 1);
+      const int cur = kt & 1;
+      store_tiles(cur ^ 1);
       // keep the loads below the stores (one register set)
       __builtin_amdgcn_sched_barrier(0);
       load_tiles(kbeg + min(kt + 2, nfull - 1) * BK, false);
-      compute(kt & 1);
-      __syncthreads();
+#pragma unroll
+      for (int st = 0; st < S; ++st) {
+        if (st + 1 < S) {
+          frag(cur, (st + 1) * 4, f[(st + 1) & 1]);
+        } else {
+          __syncthreads();
+          // after the last tile this reads the idle buffer's reloaded copy of
+          // it: in bounds, and never used
+          frag(cur ^ 1, 0, f[0]);
+        }
+        mma(f[st & 1]);
+      }
     }
+    // the tail below stores into buffer 0, which the reads just above may
+    // still be using
+    __syncthreads();
   } else {
     if (nfull > 0) {
       load_tiles(kbeg, false);
@@ -394,22 +425,22 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
 // by that shape's round fill as modelled below).
 struct Cfg { int BM, BN, BK, WM, WN, slots; float rate; };
 constexpr Cfg kCfgs[] = {
-    {128, 256, 16, 2, 4, 1, 59.8f},
-    {256, 128, 16, 4, 2, 1, 59.9f},
-    {128, 128, 16, 2, 2, 2, 66.2f},
-    {64, 128, 16, 2, 2, 2, 59.7f},
-    {128, 64, 16, 2, 2, 2, 58.0f},
-    {64, 64, 16, 2, 2, 4, 55.8f},
-    {32, 64, 16, 1, 2, 4, 42.7f},
-    {64, 32, 16, 2, 1, 4, 39.0f},
+    {128, 256, 16, 2, 4, 1, 64.9f},
+    {256, 128, 16, 4, 2, 1, 65.0f},
+    {128, 128, 16, 2, 2, 2, 68.9f},
+    {64, 128, 16, 2, 2, 2, 63.2f},
+    {128, 64, 16, 2, 2, 2, 62.4f},
+    {64, 64, 16, 2, 2, 4, 60.7f},
+    {32, 64, 16, 1, 2, 4, 44.7f},
+    {64, 32, 16, 2, 1, 4, 40.2f},
     // 48-wide tiles for the O = 40 edges (M or N of 40 fills 83% instead of 62%)
-    {128, 48, 16, 4, 1, 2, 52.7f},
-    {48, 128, 16, 1, 4, 2, 58.8f},
-    {48, 64, 16, 1, 2, 4, 45.5f},
-    {64, 48, 16, 2, 1, 4, 43.6f},
+    {128, 48, 16, 4, 1, 2, 55.4f},
+    {48, 128, 16, 1, 4, 2, 62.0f},
+    {48, 64, 16, 1, 2, 4, 50.0f},
+    {64, 48, 16, 2, 1, 4, 47.0f},
     // 112 = 7 x 16 rows: P2(40) = 780 fills 99.5% (128-row tiles: 87%)
-    {112, 128, 16, 1, 4, 2, 63.2f},
-    {128, 112, 16, 4, 1, 2, 63.0f},
+    {112, 128, 16, 1, 4, 2, 65.2f},
+    {128, 112, 16, 4, 1, 2, 64.8f},
 };
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
