@@ -93,12 +93,19 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
 
   const GemmDesc<T>& d = p.d;
   // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8, so
-  // XCD j takes the contiguous tile range [j*gx/8, (j+1)*gx/8).  With few tiles
-  // (gx not a multiple of 8) the plain order spreads the batches and K-splits
-  // over all XCDs instead of stacking every real tile on the low ones.
-  const int gx = p.tiles_pad;
-  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
-  if (tile >= p.tiles_m * p.tiles_n) return;
+  // XCD j takes the contiguous tile range [j*T/8, (j+1)*T/8) of the T tiles
+  // (6 or 7 tiles each for T = 49: no XCD is left with only padding).  With
+  // few tiles (tiles_pad not a multiple of 8) the plain order spreads the
+  // batches and K-splits over all XCDs instead of stacking every real tile on
+  // the low ones.
+  const int ntiles = p.tiles_m * p.tiles_n;
+  int tile = (int)blockIdx.x;
+  if ((p.tiles_pad & 7) == 0) {
+    const int x = (int)(blockIdx.x & 7);
+    tile = (int)(((long)x * ntiles) >> 3) + (int)(blockIdx.x >> 3);
+    if (tile >= (int)(((long)(x + 1) * ntiles) >> 3)) return;
+  }
+  if (tile >= ntiles) return;
   const int tm = tile % p.tiles_m;
   const int tn = tile / p.tiles_m;
   const int batch = blockIdx.y;
@@ -142,7 +149,54 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   // VEC: one 2-element load per pair (the host guarantees unit stride along
   // the pair, even pitches and an aligned base; M or N even when the pair
   // runs along m or n).
+  // Paired loads of an operand whose k stride is not the unit one: each
+  // thread's pair addresses at k0 = 0 are computed once, and a tile adds the
+  // uniform k0 * pitch (scalar arithmetic) instead of a per-lane 64-bit
+  // multiply.  k-contiguous operands address rows directly (row + k0 + kk).
+  const T* a_base[AK ? 1 : AIT];
+  const T* b_base[BNM ? BIT : 1];
+  if constexpr (VEC && !AK) {
+#pragma unroll
+    for (int it = 0; it < AIT; ++it) {
+      int m, kk;
+      a_unit(min(it * NT + tid, AU - 1), m, kk);
+      a_base[it] = A + (long)kk * sAk + min(m0 + m, M - 2);
+    }
+  }
+  if constexpr (VEC && BNM) {
+#pragma unroll
+    for (int it = 0; it < BIT; ++it) {
+      int n, kk;
+      b_unit(min(it * NT + tid, BU - 1), n, kk);
+      b_base[it] = B + (long)kk * sBk + min(n0 + n, N - 2);
+    }
+  }
   auto load_tiles = [&](int k0, bool checked) {
+    if (VEC && !checked) {
+#pragma unroll
+      for (int it = 0; it < AIT; ++it) {
+        const int u = it * NT + tid;
+        if (AU % NT != 0 && u >= AU) continue;
+        int m, kk;
+        a_unit(u, m, kk);
+        const T* src;
+        if constexpr (AK) src = A + (long)min(m0 + m, M - 1) * sAm + (k0 + kk);
+        else src = a_base[it] + (long)k0 * sAk;
+        ra[it] = *reinterpret_cast<const pair_t*>(src);
+      }
+#pragma unroll
+      for (int it = 0; it < BIT; ++it) {
+        const int u = it * NT + tid;
+        if (BU % NT != 0 && u >= BU) continue;
+        int n, kk;
+        b_unit(u, n, kk);
+        const T* src;
+        if constexpr (BNM) src = b_base[it] + (long)k0 * sBk;
+        else src = B + (long)min(n0 + n, N - 1) * sBn + (k0 + kk);
+        rb[it] = *reinterpret_cast<const pair_t*>(src);
+      }
+      return;
+    }
 #pragma unroll
     for (int it = 0; it < AIT; ++it) {
       const int u = it * NT + tid;
@@ -151,10 +205,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
       const int gm = m0 + m, gk = k0 + kk;
       pair_t v = {T(0), T(0)};
       if (AU % NT == 0 || u < AU) {
-        if (VEC && !checked) {
-          const T* src = AK ? A + (long)min(gm, M - 1) * sAm + gk : A + (long)gk * sAk + min(gm, M - 2);
-          v = *reinterpret_cast<const pair_t*>(src);
-        } else if (AK) {
+        if (AK) {
           const T* row = A + (long)min(gm, M - 1) * sAm;
           if (!checked || gk < kend) v[0] = row[gk * sAk];
           if (!checked || gk + 1 < kend) v[1] = row[(gk + 1) * sAk];
@@ -174,10 +225,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
       const int gn = n0 + n, gk = k0 + kk;
       pair_t v = {T(0), T(0)};
       if (BU % NT == 0 || u < BU) {
-        if (VEC && !checked) {
-          const T* src = BNM ? B + (long)gk * sBk + min(gn, N - 2) : B + (long)min(gn, N - 1) * sBn + gk;
-          v = *reinterpret_cast<const pair_t*>(src);
-        } else if (!BNM) {
+        if (!BNM) {
           const T* col = B + (long)min(gn, N - 1) * sBn;
           if (!checked || gk < kend) v[0] = col[gk * sBk];
           if (!checked || gk + 1 < kend) v[1] = col[(gk + 1) * sBk];
@@ -348,36 +396,42 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   }
 }
 
-// Many partials per output (long split-K on a small C): one workgroup per output
-// element, threads stride over the slices, fixed-order tree sum.
+// Many partials per output (long split-K on a small C): a workgroup owns 64
+// consecutive output elements; each of its 4 waves sums every 4th slice with
+// one contiguous 512-byte load per slice (4 independent chains, so several
+// loads are in flight), then the 4 waves' sums are added in a fixed order.
 template <typename T>
-__global__ void gemm_reduce_wide_kernel(GemmDesc<T> d, const T* __restrict__ W, int splits) {
+__global__ __launch_bounds__(256) void gemm_reduce_wide_kernel(GemmDesc<T> d, const T* __restrict__ W, int splits) {
   const long MN = (long)d.M * d.N;
   const int nbatch = d.nb1 * d.nb2;
-  const long e = blockIdx.x;  // element within one output matrix
-  const int bo = blockIdx.y;  // output batch (0 when reduce_batch)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long e = (long)blockIdx.x * 64 + lane;  // element within one output matrix
+  const int bo = blockIdx.y;                    // output batch (0 when reduce_batch)
   const int nslices = d.reduce_batch ? splits * nbatch : splits;
-  T s = 0;
-  for (int q = threadIdx.x; q < nslices; q += blockDim.x) {
-    long slice = d.reduce_batch ? q : (long)q * nbatch + bo;
-    s += W[slice * MN + e];
+  auto at = [&](int q) { return W[(d.reduce_batch ? (long)q : (long)q * nbatch + bo) * MN + e]; };
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (e < MN) {
+    int q = wave;
+    for (; q + 12 < nslices; q += 16) {
+      s0 += at(q);
+      s1 += at(q + 4);
+      s2 += at(q + 8);
+      s3 += at(q + 12);
+    }
+    for (; q < nslices; q += 4) s0 += at(q);
   }
-  __shared__ T sh[256];
-  sh[threadIdx.x] = s;
+  __shared__ T sh[4][64];
+  sh[wave][lane] = (s0 + s1) + (s2 + s3);
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
+  if (wave == 0 && e < MN) {
+    const T v = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
     const int row = (int)(e / d.N), col = (int)(e % d.N);
     long off = row * d.sCm + col * d.sCn;
     if (!d.reduce_batch) off += (bo / d.nb2) * d.sC1 + (bo % d.nb2) * d.sC2;
-    write_out(d, off, sh[0]);
+    write_out(d, off, v);
   }
 }
 
-// Sum partials [splits][nbatch][M][N] in a fixed order and apply alpha/beta.
 template <typename T>
 __global__ void gemm_reduce_kernel(GemmDesc<T> d, const T* __restrict__ W, int splits) {
   const long MN = (long)d.M * d.N;
@@ -549,7 +603,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   const double c_bytes = double(part_bytes_per) * (d.beta != T(0) ? 2 : 1);
   constexpr double kHbm = 6e12;  // sustained HBM bytes/s
   int best = 0, kchunk = d.K;
-  double best_t = 1e300;
+  double best_t = 1e300, best_tc = 1e300;
   for (int c = 0; c < kNumCfgs; ++c) {
     if (force_config >= 0 && force_config < kNumCfgs && c != force_config) continue;
     const Cfg& g = kCfgs[c];
@@ -557,6 +611,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     const double pad = (double(tm * g.BM) * double(tn * g.BN)) / (double(d.M) * d.N);
     const long tiles = tm * tn * nbatch;
     const double slots = double(kNumCus) * g.slots;
+    const bool xcd_order = tm * tn >= 32;  // the launch below pads to 8 and deals tiles per XCD
     const long ktiles = (d.K + g.BK - 1) / g.BK;
     const double t_hbm = (a_bytes * std::min(tn, 2L) + b_bytes * std::min(tm, 2L) + c_bytes) / kHbm;
     long s_lo = 1, s_hi = 1;
@@ -575,10 +630,16 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
       s_prev = sr;
       if (sr > 1 && sr > ws_slices) continue;
       const double units = double(tiles) * sr;
-      const double rounds = std::ceil(units / slots);
-      double t = std::max(rounds * slots / units * pad * flops / (g.rate * 1e12), t_hbm);
+      // rounds on the busiest XCD (each XCD holds 1/8 of the CUs)
+      const double per_xcd = xcd_order ? std::ceil(double(tm * tn) / 8) * double(nbatch) * sr : std::ceil(units / 8);
+      const double rounds = std::ceil(per_xcd / (slots / 8));
+      const double tc = rounds * slots / units * pad * flops / (g.rate * 1e12);
+      double t = std::max(tc, t_hbm);
       if (sr > 1 || need_partial_for_batch) t += (sr + 2) * double(part_bytes_per) / 4e12 + 5e-6;
-      if (t < best_t) { best_t = t; best = c; kchunk = kc; }
+      // at the HBM floor, the least compute (least padding) breaks the tie
+      if (t < best_t * (1 - 1e-9) || (t <= best_t * (1 + 1e-9) && tc < best_tc)) {
+        best_t = t; best_tc = tc; best = c; kchunk = kc;
+      }
     }
   }
   const Cfg& g = kCfgs[best];
@@ -606,8 +667,8 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     const int nout_b = d.reduce_batch ? 1 : nbatch;
     const long slices = d.reduce_batch ? (long)splits * nbatch : splits;
     if (slices >= 64 && MN <= (1L << 20) && nout_b <= 65535) {
-      hipLaunchKernelGGL(gemm_reduce_wide_kernel<T>, dim3((unsigned)MN, nout_b), dim3(256), 0, stream, d,
-                         (const T*)ws.ptr, splits);
+      hipLaunchKernelGGL(gemm_reduce_wide_kernel<T>, dim3((unsigned)((MN + 63) / 64), nout_b), dim3(256), 0, stream,
+                         d, (const T*)ws.ptr, splits);
     } else {
       long nout = MN * nout_b;
       int blocks = (int)std::min<long>((nout + 255) / 256, 4096);

@@ -100,7 +100,9 @@ def test_gemm_lowprec(prec, tol, M, N, K):
     assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K)
 
 
-@pytest.mark.parametrize("O,V,seed", [(4, 8, 11), (6, 10, 12), (8, 20, 13), (10, 26, 14)])
+# (12, 40) and (14, 60): enough packed pairs that the split-K and batch
+# reductions (F_mi, F_ae, the T1 tau.<ma||ef> term) take the many-slice path
+@pytest.mark.parametrize("O,V,seed", [(4, 8, 11), (6, 10, 12), (8, 20, 13), (10, 26, 14), (12, 40, 15), (14, 60, 16)])
 def test_residual_parity(O, V, seed):
     capi = _ctx()
     prob = synth.make_problem(O, V, seed, scale=0.05)
