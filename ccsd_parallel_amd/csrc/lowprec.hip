@@ -197,8 +197,10 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   constexpr int BK = 32;
   constexpr int RP = 2 * BK;  // row pitch in bf16 elements: 128 bytes, hi and lo interleaved per 8
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
-  constexpr int CA = BM * 8 / NT, CB = BN * 8 / NT;  // 16-byte pieces per thread per k-tile
-  static_assert(CA * NT == BM * 8 && CB * NT == BN * 8, "tile/threads mismatch");
+  // 16-byte pieces per thread per k-tile (the last pass partial when BM * 8 or
+  // BN * 8 is not a multiple of the thread count)
+  constexpr int CA = (BM * 8 + NT - 1) / NT, CB = (BN * 8 + NT - 1) / NT;
+  constexpr bool FA = CA * NT == BM * 8, FB = CB * NT == BN * 8;
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * RP];
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * RP];
   // LDS slot of 16-byte chunk q of tile row m
@@ -224,16 +226,18 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   int sa[CA], sb[CB];
 #pragma unroll
   for (int it = 0; it < CA; ++it) {
-    const int u = it * NT + tid, m = u >> 3, q = u & 7;
+    const int u = min(it * NT + tid, BM * 8 - 1), m = u >> 3, q = u & 7;
     pa[it] = p.A + (long)min(m0 + m, p.M - 1) * 2 * p.lda + q * 8;
     sa[it] = slot(m, q);
   }
 #pragma unroll
   for (int it = 0; it < CB; ++it) {
-    const int u = it * NT + tid, n = u >> 3, q = u & 7;
+    const int u = min(it * NT + tid, BN * 8 - 1), n = u >> 3, q = u & 7;
     pb[it] = p.B + (long)min(n0 + n, p.N - 1) * 2 * p.ldb + q * 8;
     sb[it] = slot(n, q);
   }
+  // threads past the last piece of a partial pass
+  const bool la = FA || (CA - 1) * NT + tid < BM * 8, lb = FB || (CB - 1) * NT + tid < BN * 8;
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -247,15 +251,19 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   auto load = [&](int t) {
     const long kn = 2L * min(t, nk - 1) * BK;
 #pragma unroll
-    for (int it = 0; it < CA; ++it) ra[it] = *(const u32x4*)(pa[it] + kn);
+    for (int it = 0; it < CA; ++it)
+      if (it < CA - 1 || la) ra[it] = *(const u32x4*)(pa[it] + kn);
 #pragma unroll
-    for (int it = 0; it < CB; ++it) rb[it] = *(const u32x4*)(pb[it] + kn);
+    for (int it = 0; it < CB; ++it)
+      if (it < CB - 1 || lb) rb[it] = *(const u32x4*)(pb[it] + kn);
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int it = 0; it < CA; ++it) *(u32x4*)&sA[buf][sa[it]] = ra[it];
+    for (int it = 0; it < CA; ++it)
+      if (it < CA - 1 || la) *(u32x4*)&sA[buf][sa[it]] = ra[it];
 #pragma unroll
-    for (int it = 0; it < CB; ++it) *(u32x4*)&sB[buf][sb[it]] = rb[it];
+    for (int it = 0; it < CB; ++it)
+      if (it < CB - 1 || lb) *(u32x4*)&sB[buf][sb[it]] = rb[it];
   };
   // operand fetch: lane l reads tile row 16i + (l & 15) (a multiple of 16 plus
   // l & 15, so its swizzle is ((l & 15) >> 1) & 7), k-slice l >> 4
@@ -326,11 +334,12 @@ hipError_t launch_bf16(Bf16Args p, hipStream_t s) {
 
 // C = A B^T on interleaved BF16 hi/lo operands.  Tile shape by modelled time
 // (padding x wave quantisation over 256 CUs / relative throughput measured on
-// MI355X): 256 x 256 on 8 waves for large products, 128 x 128 on 4 waves for
-// small ones.  The grouped tile order pays on square products with many tile
-// rows (measured ~3% on 12000^3) and not on the P2(O)-row ones.
+// MI355X): 256 x 256 on 8 waves for large products, 160 x 320 on 8 waves when
+// M = P2(O) = 780 (5 x 160 = 800 rows instead of 4 x 256 = 1024), 128 x 128 on
+// 4 waves for small ones.  The grouped tile order pays on square products with
+// many tile rows (measured ~3% on 12000^3) and not on the P2(O)-row ones.
 // force (tests): 0 = 128 x 128, 1 = 256 x 256 automatic order, 2 = 256 x 256
-// plain order, 3 = 256 x 256 grouped order.
+// plain order, 3 = 256 x 256 grouped order, 4 = 160 x 320.
 hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, const Bf16Out& out, int M, int N,
                      int K, hipStream_t s, int force = -1) {
   Bf16Args p;
@@ -339,10 +348,10 @@ hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, c
   p.M = M; p.N = N; p.Kp = ((K + 31) / 32) * 32;
   p.group = 0;
   struct Opt { int BM, BN; double rate; };
-  const Opt opts[2] = {{128, 128, 1.0}, {256, 256, 1.65}};
+  const Opt opts[3] = {{128, 128, 1.0}, {256, 256, 1.65}, {160, 320, 1.55}};
   int best = 0;
   double bt = 1e300;
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 3; ++c) {
     long tm = (M + opts[c].BM - 1) / opts[c].BM, tn = (N + opts[c].BN - 1) / opts[c].BN;
     double pad = double(tm * opts[c].BM) * double(tn * opts[c].BN) / (double(M) * N);
     double tiles = double(tm * tn);
@@ -357,7 +366,9 @@ hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, c
     if (force == 2) group = 0;
     if (force == 3) group = 8;
   }
+  if (force == 4) best = 2;
   if (best == 0) return launch_bf16<128, 128, 2, 2>(p, s);
+  if (best == 2) return launch_bf16<160, 320, 2, 4>(p, s);
   p.group = group;
   return launch_bf16<256, 256, 4, 2>(p, s);
 }
