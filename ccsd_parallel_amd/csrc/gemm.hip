@@ -3,9 +3,12 @@
 // Design (CDNA4-first, not a CUDA warp tiling):
 //  * 64-wide wavefronts, each computing a (BM/WM) x (BN/WN) sub-tile as a grid
 //    of 16x16 MFMA tiles: v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32.
-//  * A and B tiles are staged through LDS in k-major order ([k][m], [k][n]) so
-//    that a wave's operand fetch (lane l reads row l%16 of k-slice l/16) is a
-//    contiguous 16-element run per k; double-buffered, one barrier per k-tile.
+//  * A and B tiles are staged through LDS ([m][k] with a padded pitch for a
+//    k-contiguous operand, [k][m] otherwise) so that a wave's operand fetch
+//    (lane l reads row l%16 of k-slice l/16) puts its 16 rows in distinct
+//    banks; LDS is
+//    double-buffered with one barrier per k-tile, and the MFMA operands are
+//    double-buffered in registers (k-step s+1 read during k-step s).
 //  * Global loads are coalesced along whichever operand stride is unit
 //    (template switches AK / BNM select the loop order); any strides work.
 //    Where the strides allow, each thread loads element pairs with one
@@ -13,9 +16,9 @@
 //    only the ragged last k-tile is bounds-checked.
 //  * 4-wave tiles are compiled for two or more resident workgroups per CU, so
 //    one workgroup's barrier wait is covered by another's MFMAs.
-//  * Workgroup ids are remapped so that the tiles sharing one B column block
-//    run on the same XCD (the dispatcher deals workgroups round-robin over the
-//    8 XCDs, each with its own L2).
+//  * Workgroup ids are remapped so that each XCD (the dispatcher deals
+//    workgroups round-robin over the 8 XCDs, each with its own L2) runs a
+//    contiguous, balanced range of tiles that share operand panels.
 //  * Split-K and batch reductions write per-slice partials and are summed in a
 //    fixed order by a second kernel: results are deterministic.
 #include "gemm.hpp"
