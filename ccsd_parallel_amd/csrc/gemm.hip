@@ -474,6 +474,37 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
   if (lane == 0) write_out(d, (long)row * d.sCm, s);
 }
 
+// gemv over unit-stride rows with 16-byte alignment (the host checks): lane l
+// loads pairs k0 + 128 j + 2 l, j = 0..3, into four independent sums, so a
+// wavefront moves 2 KB of the row per iteration; the ragged end is read
+// element-wise.  The sums are combined in a fixed order.
+template <typename T>
+__global__ void gemv_pairs_kernel(GemmDesc<T> d) {
+  using pair_t = T __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= d.M) return;
+  const T* a = d.A + (long)row * d.sAm;
+  const T* x = d.B;
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int k0 = 0;
+  for (; k0 + 512 <= d.K; k0 += 512) {
+    const int k = k0 + 2 * lane;
+    const pair_t a0 = *(const pair_t*)(a + k), a1 = *(const pair_t*)(a + k + 128);
+    const pair_t a2 = *(const pair_t*)(a + k + 256), a3 = *(const pair_t*)(a + k + 384);
+    const pair_t x0 = *(const pair_t*)(x + k), x1 = *(const pair_t*)(x + k + 128);
+    const pair_t x2 = *(const pair_t*)(x + k + 256), x3 = *(const pair_t*)(x + k + 384);
+    s0 += a0[0] * x0[0] + a0[1] * x0[1];
+    s1 += a1[0] * x1[0] + a1[1] * x1[1];
+    s2 += a2[0] * x2[0] + a2[1] * x2[1];
+    s3 += a3[0] * x3[0] + a3[1] * x3[1];
+  }
+  for (int k = k0 + lane; k < d.K; k += 64) s0 += a[k] * x[k];
+  T s = (s0 + s1) + (s2 + s3);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) write_out(d, (long)row * d.sCm, s);
+}
+
 // One tile configuration: BM x BN x BK on WM x WN waves.  slots: workgroups
 // resident per CU (LDS and register limited; the kernel is compiled for at
 // least `slots` waves per SIMD when it has 4 waves, so that two or more
@@ -579,8 +610,14 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   }
   if (d.N == 1 && nbatch == 1 && force_config < 0) {
     const int rows_per_block = 4;
-    hipLaunchKernelGGL(gemv_kernel<T>, dim3((d.M + rows_per_block - 1) / rows_per_block), dim3(64 * rows_per_block),
-                       0, stream, d);
+    const bool pairs = d.sAk == 1 && d.sBk == 1 && d.sAm % 2 == 0 && d.K <= (1 << 30) &&
+                       reinterpret_cast<uintptr_t>(d.A) % 16 == 0 && reinterpret_cast<uintptr_t>(d.B) % 16 == 0;
+    if (pairs && force_pairs != 0)
+      hipLaunchKernelGGL(gemv_pairs_kernel<T>, dim3((d.M + rows_per_block - 1) / rows_per_block),
+                         dim3(64 * rows_per_block), 0, stream, d);
+    else
+      hipLaunchKernelGGL(gemv_kernel<T>, dim3((d.M + rows_per_block - 1) / rows_per_block),
+                         dim3(64 * rows_per_block), 0, stream, d);
     if (info) *info = GemmLaunchInfo{d.M, 1, 1, 1, -1, 0};
     return hipGetLastError();
   }

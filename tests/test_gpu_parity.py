@@ -37,7 +37,10 @@ def packed(t2, O, V):
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [
     (37, 53, 29, 0, 0), (128, 128, 64, 0, 1), (64, 300, 17, 1, 0), (200, 33, 1000, 1, 1), (1, 77, 500, 0, 0),
-    (190, 257, 4000, 0, 1), (5, 5, 20000, 0, 0), (300, 1, 300, 0, 0)])
+    (190, 257, 4000, 0, 1), (5, 5, 20000, 0, 0), (300, 1, 300, 0, 0),
+    # N = 1: pair loads with full 512-element chunks and a ragged end; an odd
+    # row pitch (element loads); a transposed A (element loads)
+    (257, 1, 1500, 0, 0), (301, 1, 301, 0, 0), (64, 1, 700, 1, 0)])
 def test_gemm_fp64(M, N, K, ta, tb):
     capi = _ctx()
     rng = np.random.default_rng(M * 7 + N)
