@@ -120,14 +120,18 @@ __global__ void build_oovv_vu_kernel(double* U, const double* oovv, Dims d) {
 }
 
 // ---------------------------------------------------------------- amplitude layouts
+// Row ij = blockIdx.y (+ multiples of gridDim.y), columns ab over x: the row's
+// (i, j) is decoded once per block, and no element needs a 64-bit division.
 __global__ void build_tau_kernel(double* tau, const double* t2, const double* t1, Dims d) {
-  const long n = d.po * d.pv;
-  GRID_LOOP(idx, n) {
-    long ij = idx / d.pv, ab = idx % d.pv;
-    int i, j, a, b;
+  const long ab = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (ab >= d.pv) return;
+  int a, b;
+  pair_decode(ab, a, b);
+  for (long ij = blockIdx.y; ij < d.po; ij += gridDim.y) {
+    int i, j;
     pair_decode(ij, i, j);
-    pair_decode(ab, a, b);
-    tau[idx] = t2[idx] + t1[(long)i * d.V + a] * t1[(long)j * d.V + b] - t1[(long)i * d.V + b] * t1[(long)j * d.V + a];
+    const long e = ij * d.pv + ab;
+    tau[e] = t2[e] + t1[(long)i * d.V + a] * t1[(long)j * d.V + b] - t1[(long)i * d.V + b] * t1[(long)j * d.V + a];
   }
 }
 
@@ -497,13 +501,26 @@ __global__ void maxabs_partials_kernel(double* partial, const double* x, long n)
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Partial dot products of y with nv vectors, 8 vectors per pass over y (y is
+// read once per 8 vectors, not once per vector).  Each sum accumulates in the
+// same per-thread order as a single dot product would.
 __global__ void multidot_partials_kernel(double* partial, const double* X, long ldx, int nv, const double* y, long n) {
-  for (int v = 0; v < nv; ++v) {
-    const double* x = X + (long)v * ldx;
-    double s = 0.0;
-    GRID_LOOP(i, n) s += x[i] * y[i];
-    s = block_sum(s);
-    if (threadIdx.x == 0) partial[(long)v * gridDim.x + blockIdx.x] = s;
+  for (int v0 = 0; v0 < nv; v0 += 8) {
+    const int cnt = min(8, nv - v0);
+    const double* x = X + (long)v0 * ldx;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    GRID_LOOP(i, n) {
+      const double yi = y[i];
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        if (v < cnt) s[v] += x[(long)v * ldx + i] * yi;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      if (v >= cnt) break;
+      const double t = block_sum(s[v]);
+      if (threadIdx.x == 0) partial[(long)(v0 + v) * gridDim.x + blockIdx.x] = t;
+    }
   }
 }
 
@@ -653,7 +670,8 @@ void k_build_oovv_vu(double* U, const double* oovv, Dims d, hipStream_t s) {
 }
 
 void k_build_tau(double* tau, const double* t2, const double* t1, Dims d, hipStream_t s) {
-  LAUNCH(build_tau_kernel, d.po * d.pv, tau, t2, t1, d);
+  const dim3 grid((unsigned)((d.pv + kThreads - 1) / kThreads), (unsigned)std::min<long>(d.po, 65535));
+  hipLaunchKernelGGL(build_tau_kernel, grid, dim3(kThreads), 0, s, tau, t2, t1, d);
 }
 void k_unpack_occ(double* T2u, double* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
   LAUNCH(unpack_occ_kernel<double>, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
