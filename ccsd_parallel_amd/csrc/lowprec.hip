@@ -269,25 +269,26 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   // l & 15, so its swizzle is ((l & 15) >> 1) & 7), k-slice l >> 4
   const int swz = (lane >> 1) & 7, ks = lane >> 4;
   const int oh = ((2 * ks) ^ swz) << 3, ol = ((2 * ks + 1) ^ swz) << 3;
-  auto compute = [&](int buf) {
-    bf16x8_t ah[TM], al[TM];
+  struct AFrag { bf16x8_t h[TM], l[TM]; };
+  auto read_a = [&](int buf, AFrag& f) {
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int r = (wm0 + i * 16 + (lane & 15)) * RP;
-      ah[i] = *(const bf16x8_t*)&sA[buf][r + oh];
-      al[i] = *(const bf16x8_t*)&sA[buf][r + ol];
+      f.h[i] = *(const bf16x8_t*)&sA[buf][r + oh];
+      f.l[i] = *(const bf16x8_t*)&sA[buf][r + ol];
     }
+  };
+  auto read_b = [&](int buf, int j, bf16x8_t& bh, bf16x8_t& bl) {
+    const int r = (wn0 + j * 16 + (lane & 15)) * RP;
+    bh = *(const bf16x8_t*)&sB[buf][r + oh];
+    bl = *(const bf16x8_t*)&sB[buf][r + ol];
+  };
+  auto mma = [&](const AFrag& f, int j, bf16x8_t bh, bf16x8_t bl) {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int r = (wn0 + j * 16 + (lane & 15)) * RP;
-      const bf16x8_t bh = *(const bf16x8_t*)&sB[buf][r + oh];
-      const bf16x8_t bl = *(const bf16x8_t*)&sB[buf][r + ol];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
-      }
+    for (int i = 0; i < TM; ++i) {
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.l[i], bh, acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.h[i], bl, acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.h[i], bh, acc[i][j], 0, 0, 0);
     }
   };
 
@@ -295,21 +296,43 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   // iteration) into the idle buffer, which the previous barrier released,
   // then issues the loads of tile kt+2 and computes tile kt: the LDS stores
   // overlap the MFMAs instead of sitting between the last MFMA and the
-  // barrier.  (A second register stage does not fit the 256 registers of 2
-  // waves per SIMD.)
-  load(0);
-  store(0);
-  load(1);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    store((kt & 1) ^ 1);
+  // barrier.  The tile's one barrier sits before its last column block, whose
+  // B operand is read before it (the buffer is refilled behind the barrier):
+  // behind the barrier each wave reads tile kt+1's A operands into the other
+  // register set while the last MFMAs of tile kt run.  (A second global-load
+  // register stage does not fit the 256 registers of 2 waves per SIMD.)
+  auto step = [&](int kt, const AFrag& cur, AFrag& nxt) {
+    const int buf = kt & 1;
+    store(buf ^ 1);
     // keep the loads below the stores: hoisted above them, the new tile would
     // need a second register set while the old one still waits to be stored
     __builtin_amdgcn_sched_barrier(0);
     load(kt + 2);
-    compute(kt & 1);
+    bf16x8_t bh, bl;
+#pragma unroll
+    for (int j = 0; j < TN - 1; ++j) {
+      read_b(buf, j, bh, bl);
+      mma(cur, j, bh, bl);
+    }
+    read_b(buf, TN - 1, bh, bl);
     __syncthreads();
+    // after the last tile this reads the idle buffer's reloaded copy of it:
+    // in bounds, and never used
+    read_a(buf ^ 1, nxt);
+    mma(cur, TN - 1, bh, bl);
+  };
+  AFrag f0, f1;
+  load(0);
+  store(0);
+  load(1);
+  __syncthreads();
+  read_a(0, f0);
+  int kt = 0;
+  for (; kt + 2 <= nk; kt += 2) {
+    step(kt, f0, f1);
+    step(kt + 1, f1, f0);
   }
+  if (kt < nk) step(kt, f0, f1);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
