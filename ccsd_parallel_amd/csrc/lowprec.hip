@@ -365,6 +365,9 @@ hipError_t launch_bf16(Bf16Args p, hipStream_t s) {
 // plain order, 3 = 256 x 256 grouped order, 4 = 160 x 320.
 hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, const Bf16Out& out, int M, int N,
                      int K, hipStream_t s, int force = -1) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  // the kernel clamps its k-tile index to [0, K/32 - 1]: it needs one tile
+  if (K <= 0) return hipErrorInvalidValue;
   Bf16Args p;
   p.A = A; p.B = B; p.out = out;
   p.lda = lda; p.ldb = ldb;
