@@ -72,6 +72,14 @@ struct KArgs {
   // used; equal to tiles (not a multiple of 8) for the plain order
   int tiles_m, tiles_n, tiles_pad;
   int splits, kchunk;
+  // Split tail (tail_split > 0; one batch, no split-K): workgroup x runs on
+  // XCD x % 8; each XCD first takes xcd_full whole tiles of the contiguous
+  // range [x * xcd_full, (x + 1) * xcd_full), then units of the tail tiles
+  // [8 * xcd_full, tiles), each cut into tail_split k-chunks of tail_kchunk,
+  // dealt round-robin over the XCDs.  A tail unit writes its BM x BN partial
+  // to tail_partial[unit]; a second kernel sums them in a fixed order.
+  int tail_split, xcd_full, tail_units, tail_kchunk;
+  T* tail_partial;
 };
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool AK, bool BNM, int WPE, bool VEC>
@@ -102,8 +110,25 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   // batches and K-splits over all XCDs instead of stacking every real tile on
   // the low ones.
   const int ntiles = p.tiles_m * p.tiles_n;
+  const int batch = blockIdx.y;
+  const int split = blockIdx.z;
   int tile = (int)blockIdx.x;
-  if ((p.tiles_pad & 7) == 0) {
+  int kbeg = split * p.kchunk;
+  int kend = min(d.K, kbeg + p.kchunk);
+  T* tail_out = nullptr;  // this unit's compact partial (split tail tiles)
+  if (p.tail_split > 0) {
+    const int x = (int)(blockIdx.x & 7), r = (int)(blockIdx.x >> 3);
+    if (r < p.xcd_full) {
+      tile = x * p.xcd_full + r;
+    } else {
+      const int v = (r - p.xcd_full) * 8 + x;
+      if (v >= p.tail_units) return;
+      tile = 8 * p.xcd_full + v / p.tail_split;
+      kbeg = (v % p.tail_split) * p.tail_kchunk;
+      kend = min(d.K, kbeg + p.tail_kchunk);
+      tail_out = p.tail_partial + (long)v * (BM * BN);
+    }
+  } else if ((p.tiles_pad & 7) == 0) {
     const int x = (int)(blockIdx.x & 7);
     tile = (int)(((long)x * ntiles) >> 3) + (int)(blockIdx.x >> 3);
     if (tile >= (int)(((long)(x + 1) * ntiles) >> 3)) return;
@@ -111,15 +136,11 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   if (tile >= ntiles) return;
   const int tm = tile % p.tiles_m;
   const int tn = tile / p.tiles_m;
-  const int batch = blockIdx.y;
-  const int split = blockIdx.z;
   const int b1 = batch / d.nb2, b2 = batch % d.nb2;
 
   const T* __restrict__ A = d.A + b1 * d.sA1 + b2 * d.sA2;
   const T* __restrict__ B = d.B + b1 * d.sB1 + b2 * d.sB2;
   const int m0 = tm * BM, n0 = tn * BN;
-  const int kbeg = split * p.kchunk;
-  const int kend = min(d.K, kbeg + p.kchunk);
   const int M = d.M, N = d.N;
   const long sAm = d.sAm, sAk = d.sAk, sBk = d.sBk, sBn = d.sBn;
 
@@ -370,7 +391,20 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   }
 
   // Epilogue (output row of each accumulator element from the MFMA trait).
-  if (p.partial) {
+  if (tail_out) {
+    // whole BM x BN tile in tile coordinates; the reduction drops rows and
+    // columns past M and N
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm0 + i * 16 + Mfma<T>::out_row(lane, r);
+          const int col = wn0 + j * 16 + (lane & 15);
+          tail_out[row * BN + col] = acc[i][j][r];
+        }
+  } else if (p.partial) {
     const long MN = (long)d.M * d.N;
     const int nbatch = d.nb1 * d.nb2;
     T* W = p.partial + ((long)split * nbatch + batch) * MN;
@@ -472,6 +506,25 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
   for (int k = lane; k < d.K; k += 64) s += a[k * d.sAk] * d.B[k * d.sBk];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) write_out(d, (long)row * d.sCm, s);
+}
+
+// Split tail tiles: C(tile) = alpha * sum_s W[tile][s] + beta * C, the splits
+// summed in order; W holds whole bm x bn tiles, rows and columns past M and N
+// are dropped here.
+template <typename T>
+__global__ void gemm_tail_reduce_kernel(GemmDesc<T> d, const T* __restrict__ W, int first_tile, int tiles_m,
+                                        int ntail, int nsplit, int bm, int bn) {
+  const long per = (long)bm * bn;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < ntail * per;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long tt = idx / per, e = idx % per;
+    const int tile = first_tile + (int)tt;
+    const int row = (tile % tiles_m) * bm + (int)(e / bn), col = (tile / tiles_m) * bn + (int)(e % bn);
+    if (row >= d.M || col >= d.N) continue;
+    T s = 0;
+    for (int sp = 0; sp < nsplit; ++sp) s += W[(tt * nsplit + sp) * per + e];
+    write_out(d, (long)row * d.sCm + (long)col * d.sCn, s);
+  }
 }
 
 // gemv over unit-stride rows with 16-byte alignment (the host checks): lane l
@@ -690,6 +743,35 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   const bool use_partial = splits > 1 || need_partial_for_batch;
   if (use_partial && (long)splits > ws_slices) return hipErrorOutOfMemory;
 
+  // Split tail.  With whole-K tiles the last round on each XCD is partly
+  // empty (12000^3 in 128 x 128 tiles: 1104.5 tiles per XCD on 64 slots, 18
+  // rounds for 17.3 of work; measured 5% slower per flop than a shape of 17
+  // full rounds).  The tiles past the last full round are cut into k-chunks
+  // so that the tail costs a fraction of a round, then reduced in order.
+  int tail_split = 0, tail_kchunk = 0;
+  long xcd_full = 0, tail = 0;
+  {
+    const long slots_xcd = (long)(kNumCus / 8) * g.slots;
+    xcd_full = (tiles / 8) / slots_xcd * slots_xcd;  // whole tiles per XCD
+    tail = tiles - 8 * xcd_full;
+    if (splits == 1 && nbatch == 1 && !d.reduce_batch && force_split <= 0 && tiles >= 32 && xcd_full > 0 &&
+        tail > 0) {
+      const double r0 = std::ceil(std::ceil(double(tiles) / 8) / slots_xcd);
+      double best_r = 0.99 * r0;  // at least 1% fewer rounds
+      for (int sp = 2; sp <= 8; ++sp) {
+        int kc = (d.K + sp - 1) / sp;
+        kc = ((kc + g.BK - 1) / g.BK) * g.BK;
+        const int se = (d.K + kc - 1) / kc;
+        if (se < 2 || kc / g.BK < 8) continue;
+        if (double(tail) * se * g.BM * g.BN * sizeof(T) > double(ws.bytes)) continue;
+        const double r = double(xcd_full) / slots_xcd +
+                         std::ceil(std::ceil(double(tail * se) / 8) / slots_xcd) / se + 0.01 * se;
+        if (r < best_r) { best_r = r; tail_split = se; tail_kchunk = kc; }
+      }
+    }
+    if (tail_split == 0) xcd_full = tail = 0;
+  }
+
   KArgs<T> ka;
   ka.d = d;
   ka.partial = use_partial ? (T*)ws.ptr : nullptr;
@@ -698,10 +780,23 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   ka.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
   ka.splits = splits;
   ka.kchunk = kchunk;
+  ka.tail_split = tail_split;
+  ka.xcd_full = (int)xcd_full;
+  ka.tail_units = (int)(tail * tail_split);
+  ka.tail_kchunk = tail_kchunk;
+  ka.tail_partial = tail_split ? (T*)ws.ptr : nullptr;
   if (ka.tiles_pad <= 0 || splits > 65535) return hipErrorInvalidValue;
   dim3 grid(ka.tiles_pad, nbatch, splits);
+  if (tail_split) grid = dim3((unsigned)(8 * (xcd_full + (ka.tail_units + 7) / 8)), 1, 1);
   hipError_t e = launch_any<T>(best, ka, AK, BNM, vec, grid, stream);
   if (e != hipSuccess) return e;
+  if (tail_split) {
+    const long n = tail * g.BM * g.BN;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(gemm_tail_reduce_kernel<T>, dim3(blocks), dim3(256), 0, stream, d, (const T*)ws.ptr,
+                       (int)(8 * xcd_full), tiles_m, (int)tail, tail_split, g.BM, g.BN);
+    e = hipGetLastError();
+  }
   if (use_partial) {
     const long MN = (long)d.M * d.N;
     const int nout_b = d.reduce_batch ? 1 : nbatch;
@@ -716,7 +811,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     }
     e = hipGetLastError();
   }
-  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best, vec ? 1 : 0};
+  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best, vec ? 1 : 0, tail_split};
   return e;
 }
 
