@@ -699,9 +699,10 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
   }
   if (rc == CCSD_OK) {
     if ((precision & 0xff) == CCSD_PREC_FP64) {
-      (void)hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice);
-      (void)hipMemcpy(dB, B, (size_t)K * N * 8, hipMemcpyHostToDevice);
-      (void)hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice);
+      if (hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(dB, B, (size_t)K * N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = CCSD_ERR_HIP;
       GemmDesc<double> g;
       g.A = dA; g.B = dB; g.C = dC; g.M = M; g.N = N; g.K = K;
       g.sAm = ta ? 1 : K; g.sAk = ta ? M : 1;
@@ -710,7 +711,8 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
       g.alpha = alpha; g.beta = beta;
       const int cfg = ((precision >> 8) & 0xff) - 1;  // forced tile configuration, -1 = automatic
       const int pairs = (precision >> 16) & 1 ? 0 : -1;  // bit 16: element-wise loads only
-      if (gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, 0, pairs) != hipSuccess) rc = CCSD_ERR_HIP;
+      if (rc == CCSD_OK && gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, 0, pairs) != hipSuccess)
+        rc = CCSD_ERR_HIP;
     } else {
       // low-precision path wants A as [M][K] and B as [N][K]
       std::vector<double> Ak((size_t)M * K), Bk((size_t)N * K);
@@ -718,10 +720,12 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
         for (long k = 0; k < K; ++k) Ak[m * K + k] = ta ? A[k * M + m] : A[m * K + k];
       for (long n = 0; n < N; ++n)
         for (long k = 0; k < K; ++k) Bk[n * K + k] = tb ? B[n * K + k] : B[k * N + n];
-      (void)hipMemcpy(dA, Ak.data(), (size_t)M * K * 8, hipMemcpyHostToDevice);
-      (void)hipMemcpy(dB, Bk.data(), (size_t)K * N * 8, hipMemcpyHostToDevice);
-      (void)hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice);
-      rc = lowprec_gemm_nt(dA, dB, dC, M, N, K, alpha, beta, precision, s);
+      if (hipMemcpy(dA, Ak.data(), (size_t)M * K * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(dB, Bk.data(), (size_t)K * N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(dC, C, (size_t)M * N * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = CCSD_ERR_HIP;
+      else
+        rc = lowprec_gemm_nt(dA, dB, dC, M, N, K, alpha, beta, precision, s);
     }
   }
   if (rc == CCSD_OK) {
