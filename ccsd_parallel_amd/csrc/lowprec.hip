@@ -150,7 +150,29 @@ struct Bf16Args {
   int M, N, Kp;           // Kp: K rounded up to 32 (operands zero padded)
   int tiles_m, tiles_n, tiles_pad, group;
   Bf16Out out;
+  // Split tail (tail_split > 0): workgroup x runs on XCD x % 8; each XCD first
+  // takes xcd_full whole tiles of the range [x * xcd_full, (x + 1) * xcd_full),
+  // then units of the tail tiles [8 * xcd_full, tiles), each cut into
+  // tail_split chunks of tail_kt k-tiles, dealt round-robin over the XCDs.  A
+  // tail unit writes its BM x BN FP32 partial to tail_partial[unit]; a second
+  // kernel sums them in a fixed order.
+  int tail_split = 0, xcd_full = 0, tail_units = 0, tail_kt = 0;
+  float* tail_partial = nullptr;
 };
+
+// Tile index -> (tile row, tile column): column-major, or walked in groups of
+// `group` tile rows.  A bijection over all tiles; the split-tail reduction
+// uses the same map.
+__device__ __forceinline__ void bf16_tile_coords(const Bf16Args& p, int tile, int& tm, int& tn) {
+  tm = tile % p.tiles_m;
+  tn = tile / p.tiles_m;
+  if (p.group > 0) {
+    const int per = p.group * p.tiles_n, g = tile / per, first = g * p.group;
+    const int gs = min(p.tiles_m - first, p.group), r = tile % per;
+    tm = first + r % gs;
+    tn = r / gs;
+  }
+}
 
 __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, float acc) {
   if (o.C64) {
@@ -206,16 +228,27 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   // LDS slot of 16-byte chunk q of tile row m
   auto slot = [](int m, int q) { return m * RP + ((q ^ ((m >> 1) & 7)) << 3); };
 
-  const int gx = p.tiles_pad;
-  const int tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
-  if (tile >= p.tiles_m * p.tiles_n) return;
-  int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
-  if (p.group > 0) {
-    const int per = p.group * p.tiles_n, g = tile / per, first = g * p.group;
-    const int gs = min(p.tiles_m - first, p.group), r = tile % per;
-    tm = first + r % gs;
-    tn = r / gs;
+  int tile, kt0 = 0, kt1 = p.Kp / 32;  // k-tile range of this unit
+  float* tail_out = nullptr;           // this unit's FP32 partial (split tail tiles)
+  if (p.tail_split > 0) {
+    const int x = (int)(blockIdx.x & 7), r = (int)(blockIdx.x >> 3);
+    if (r < p.xcd_full) {
+      tile = x * p.xcd_full + r;
+    } else {
+      const int v = (r - p.xcd_full) * 8 + x;
+      if (v >= p.tail_units) return;
+      tile = 8 * p.xcd_full + v / p.tail_split;
+      kt0 = (v % p.tail_split) * p.tail_kt;
+      kt1 = min(kt1, kt0 + p.tail_kt);
+      tail_out = p.tail_partial + (long)v * (BM * BN);
+    }
+  } else {
+    const int gx = p.tiles_pad;
+    tile = (gx & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3));
   }
+  if (tile >= p.tiles_m * p.tiles_n) return;
+  int tm, tn;
+  bf16_tile_coords(p, tile, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
@@ -244,12 +277,12 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
-  const int nk = p.Kp / BK;
+  const int nk = kt1 - kt0;
   // global -> registers for k-tile t (clamped: the tail reloads the last tile into the idle buffer)
   typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
   u32x4 ra[CA], rb[CB];
   auto load = [&](int t) {
-    const long kn = 2L * min(t, nk - 1) * BK;
+    const long kn = 2L * (kt0 + min(t, nk - 1)) * BK;
 #pragma unroll
     for (int it = 0; it < CA; ++it)
       if (it < CA - 1 || la) ra[it] = *(const u32x4*)(pa[it] + kn);
@@ -333,6 +366,18 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
     step(kt + 1, f1, f0);
   }
   if (kt < nk) step(kt, f0, f1);
+  if (tail_out) {
+    // whole BM x BN tile in tile coordinates; the reduction drops rows and
+    // columns past M and N
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          tail_out[(wm0 + i * 16 + 4 * (lane >> 4) + r) * BN + wn0 + j * 16 + (lane & 15)] = acc[i][j][r];
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -345,13 +390,70 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
       }
 }
 
+// Split tail tiles: out(tile) = sum_s partial[tile][s], the chunks summed in
+// order (FP32, as the kernel accumulates), through the same output path.
+template <int BM, int BN>
+__global__ void bf16_tail_reduce_kernel(Bf16Args p, int ntail) {
+  constexpr long per = (long)BM * BN;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < ntail * per;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long tt = idx / per, e = idx % per;
+    int tm, tn;
+    bf16_tile_coords(p, 8 * p.xcd_full + (int)tt, tm, tn);
+    const int row = tm * BM + (int)(e / BN), col = tn * BN + (int)(e % BN);
+    if (row >= p.M || col >= p.N) continue;
+    float v = 0.0f;
+    for (int sp = 0; sp < p.tail_split; ++sp) v += p.tail_partial[(tt * p.tail_split + sp) * per + e];
+    bf16_store(p.out, row, col, v);
+  }
+}
+
+// Launch with the split tail when a workspace is given and it saves at least
+// 1% of the modelled rounds: the tiles past the last full round of every XCD
+// run as k-chunks (at least 8 k-tiles each), reduced afterwards.
 template <int BM, int BN, int WM, int WN>
-hipError_t launch_bf16(Bf16Args p, hipStream_t s) {
+hipError_t launch_bf16(Bf16Args p, hipStream_t s, const GemmWorkspace* ws) {
   p.tiles_m = (p.M + BM - 1) / BM;
   p.tiles_n = (p.N + BN - 1) / BN;
   const long tiles = (long)p.tiles_m * p.tiles_n;
   p.tiles_pad = tiles >= 32 ? (int)(((tiles + 7) / 8) * 8) : (int)tiles;
-  hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, WM, WN>), dim3(p.tiles_pad), dim3(64 * WM * WN), 0, s, p);
+  long tail = 0;
+  if (ws && ws->ptr) {
+    constexpr long kCusPerXcd = 32;  // MI355X: 256 CUs in 8 XCDs
+    constexpr long slots_xcd = kCusPerXcd * ((WM * WN == 4) ? 2 : 1);  // resident blocks per XCD
+    const long xcd_full = (tiles / 8) / slots_xcd * slots_xcd;
+    const long rest = tiles - 8 * xcd_full;
+    const int nk = p.Kp / 32;
+    if (rest > 0) {
+      const double r0 = std::ceil(std::ceil(double(tiles) / 8) / slots_xcd);
+      double best_r = 0.99 * r0;
+      for (int sp = 2; sp <= 8; ++sp) {
+        const int kt = (nk + sp - 1) / sp, se = (nk + kt - 1) / kt;
+        if (se < 2 || kt < 8) continue;
+        if (double(rest) * se * BM * BN * sizeof(float) > double(ws->bytes)) continue;
+        const double r = double(xcd_full) / slots_xcd +
+                         std::ceil(std::ceil(double(rest * se) / 8) / slots_xcd) / se + 0.01 * se;
+        if (r < best_r) {
+          best_r = r;
+          p.tail_split = se;
+          p.tail_kt = kt;
+        }
+      }
+      if (p.tail_split > 0) {
+        tail = rest;
+        p.xcd_full = (int)xcd_full;
+        p.tail_units = (int)(rest * p.tail_split);
+        p.tail_partial = (float*)ws->ptr;
+      }
+    }
+  }
+  const unsigned grid = p.tail_split ? (unsigned)(8 * (p.xcd_full + (p.tail_units + 7) / 8)) : (unsigned)p.tiles_pad;
+  hipLaunchKernelGGL((bf16x3_nt_kernel<BM, BN, WM, WN>), dim3(grid), dim3(64 * WM * WN), 0, s, p);
+  if (tail > 0) {
+    const long n = tail * BM * BN;
+    hipLaunchKernelGGL((bf16_tail_reduce_kernel<BM, BN>), dim3((unsigned)std::min<long>((n + 255) / 256, 4096)),
+                       dim3(256), 0, s, p, (int)tail);
+  }
   return hipGetLastError();
 }
 
@@ -364,7 +466,7 @@ hipError_t launch_bf16(Bf16Args p, hipStream_t s) {
 // force (tests): 0 = 128 x 128, 1 = 256 x 256 automatic order, 2 = 256 x 256
 // plain order, 3 = 256 x 256 grouped order, 4 = 160 x 320.
 hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, const Bf16Out& out, int M, int N,
-                     int K, hipStream_t s, int force = -1) {
+                     int K, hipStream_t s, int force = -1, const GemmWorkspace* ws = nullptr) {
   if (M <= 0 || N <= 0) return hipSuccess;
   // the kernel clamps its k-tile index to [0, K/32 - 1]: it needs one tile
   if (K <= 0) return hipErrorInvalidValue;
@@ -393,10 +495,10 @@ hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, c
     if (force == 3) group = 8;
   }
   if (force == 4) best = 2;
-  if (best == 0) return launch_bf16<128, 128, 2, 2>(p, s);
-  if (best == 2) return launch_bf16<160, 320, 2, 4>(p, s);
+  if (best == 0) return launch_bf16<128, 128, 2, 2>(p, s, ws);
+  if (best == 2) return launch_bf16<160, 320, 2, 4>(p, s, ws);
   p.group = group;
-  return launch_bf16<256, 256, 4, 2>(p, s);
+  return launch_bf16<256, 256, 4, 2>(p, s, ws);
 }
 
 }  // namespace
@@ -596,7 +698,7 @@ static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, lo
     Bf16Out o;
     o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
     o.Ob = Ob; o.ldo = ldo;
-    e = bf16x3_nt(Ab, lda, Bb, ldb, o, M, N, K, c->stream);
+    e = bf16x3_nt(Ab, lda, Bb, ldb, o, M, N, K, c->stream, -1, &c->ws);
   }
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
@@ -645,7 +747,7 @@ static int bf16_nt(Ctx* c, const uint16_t* A, long lda, const uint16_t* B, long 
                    double* C, long ldc, double alpha, double beta) {
   Bf16Out o;
   o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
-  hipError_t e = bf16x3_nt(A, lda, B, ldb, o, (int)M, (int)N, (int)K, c->stream);
+  hipError_t e = bf16x3_nt(A, lda, B, ldb, o, (int)M, (int)N, (int)K, c->stream, -1, &c->ws);
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;
@@ -745,7 +847,8 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
     if (e == hipSuccess) e = al((void**)&Bb, (size_t)N * ld * 4);
   }
   void* wsp = nullptr;
-  if (e == hipSuccess) e = al(&wsp, 16u << 20);
+  constexpr size_t kWs = 64u << 20;
+  if (e == hipSuccess) e = al(&wsp, kWs);
   if (e == hipSuccess) {
     if (prec == CCSD_PREC_FP32) {
       hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(M * ld)), dim3(kThreads), 0, s, A32, A, (long)M, (long)K,
@@ -757,7 +860,7 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
       g.M = M; g.N = N; g.K = K;
       g.sAm = ld; g.sAk = 1; g.sBk = 1; g.sBn = ld; g.sCm = N; g.sCn = 1;
       g.alpha = (float)alpha; g.beta = (float)beta;
-      GemmWorkspace ws{wsp, 16u << 20};
+      GemmWorkspace ws{wsp, kWs};
       e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
     } else {
       hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(M * ld / 8)), dim3(kThreads), 0, s, Ab, A, (long)M,
@@ -766,7 +869,8 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
                          (long)K, (long)K, ld, 1.0);
       Bf16Out o;
       o.C64 = C; o.ldc = N; o.alpha = alpha; o.beta = beta;
-      e = bf16x3_nt(Ab, ld, Bb, ld, o, M, N, K, s, tile - 1);
+      GemmWorkspace ws{wsp, kWs};
+      e = bf16x3_nt(Ab, ld, Bb, ld, o, M, N, K, s, tile - 1, &ws);
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
