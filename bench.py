@@ -233,8 +233,8 @@ def main():
     ap.add_argument("--no-rhf", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU; the multi-GPU sharding is a design note (DESIGN.md section 7)",
-              file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): this build has no multi-GPU path; "
+              "the sharding across GPUs is a design only", file=sys.stderr)
         sys.exit(2)
 
     from ccsd_parallel_amd import capi
