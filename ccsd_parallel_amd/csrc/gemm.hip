@@ -564,7 +564,17 @@ __global__ void gemv_pairs_kernel(GemmDesc<T> d) {
 // workgroups overlap each other's barriers).  rate: FP64 TF/s with every slot
 // busy and paired loads (12000^3 GEMM on MI355X, tile-sweep probe, divided
 // by that shape's round fill as modelled below).
-struct Cfg { int BM, BN, BK, WM, WN, slots; float rate; };
+// pairs_only / k_contig_b_only: variants that exceed their register budget
+// (spill) with element loads / with an n-contiguous B are left out of the
+// automatic choice (forced configurations still run them, correctly).
+// f64_only: measured slower than the 4-wave tiles in FP32 (the FP32 MFMA takes
+// half the cycles, so the extra LDS reads per MFMA of the smaller wave tiles
+// show), left out of the FP32 choice.
+struct Cfg {
+  int BM, BN, BK, WM, WN, slots;
+  float rate;
+  bool pairs_only = false, k_contig_b_only = false, f64_only = false;
+};
 constexpr Cfg kCfgs[] = {
     {128, 256, 16, 2, 4, 1, 64.9f},
     {256, 128, 16, 4, 2, 1, 65.0f},
@@ -582,6 +592,15 @@ constexpr Cfg kCfgs[] = {
     // 112 = 7 x 16 rows: P2(40) = 780 fills 99.5% (128-row tiles: 87%)
     {112, 128, 16, 1, 4, 2, 65.2f},
     {128, 112, 16, 4, 1, 2, 64.8f},
+    // 8 waves, two workgroups per CU: 4 waves per SIMD (128 registers each)
+    // cover each other's LDS and barrier waits (FP64 12000^3: 69-70 TF/s
+    // against 67-68 for the 4-wave 128 x 128; FP64 iteration 183.5 -> 180.2
+    // ms, same-box A/B)
+    {128, 128, 16, 4, 2, 2, 72.6f, true, false, true},
+    {128, 128, 16, 2, 4, 2, 72.4f, true, false, true},
+    // paired loads of an n-contiguous B need more than its 128 registers
+    // (spills), so it serves k-contiguous B only (W_mbij: 61.1 TF/s)
+    {112, 128, 16, 1, 8, 2, 67.0f, true, true, true},
 };
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
@@ -603,9 +622,9 @@ hipError_t launch_layout(const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipSt
 template <typename T, int C>
 hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, bool vec, dim3 grid, hipStream_t s) {
   constexpr Cfg g = kCfgs[C];
-  // waves-per-SIMD floor handed to the compiler: 4-wave tiles with 2+ slots
-  // must fit 512/slots registers
-  constexpr int wpe = (g.WM * g.WN == 4) ? g.slots : 1;
+  // waves-per-SIMD floor handed to the compiler: a CU's 4 SIMDs hold slots x
+  // (WM x WN) waves, which must fit 512 registers per SIMD lane between them
+  constexpr int wpe = (g.WM * g.WN == 4) ? g.slots : (g.WM * g.WN == 8 && g.slots == 2) ? 4 : 1;
   if (vec) return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, true>(ka, AK, BNM, grid, s);
   return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, false>(ka, AK, BNM, grid, s);
 }
@@ -626,10 +645,13 @@ hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, bool vec, di
     case 10: return launch_cfg<T, 10>(ka, AK, BNM, vec, grid, s);
     case 11: return launch_cfg<T, 11>(ka, AK, BNM, vec, grid, s);
     case 12: return launch_cfg<T, 12>(ka, AK, BNM, vec, grid, s);
-    default: return launch_cfg<T, 13>(ka, AK, BNM, vec, grid, s);
+    case 13: return launch_cfg<T, 13>(ka, AK, BNM, vec, grid, s);
+    case 14: return launch_cfg<T, 14>(ka, AK, BNM, vec, grid, s);
+    case 15: return launch_cfg<T, 15>(ka, AK, BNM, vec, grid, s);
+    default: return launch_cfg<T, 16>(ka, AK, BNM, vec, grid, s);
   }
 }
-static_assert(kNumCfgs == 14, "launch_any covers every configuration");
+static_assert(kNumCfgs == 17, "launch_any covers every configuration");
 
 // Paired (2-element vector) loads are legal when each operand has unit stride
 // along its pair index, every other stride and batch stride is even, the base
@@ -700,6 +722,8 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   for (int c = 0; c < kNumCfgs; ++c) {
     if (force_config >= 0 && force_config < kNumCfgs && c != force_config) continue;
     const Cfg& g = kCfgs[c];
+    if (force_config < 0 && ((g.pairs_only && !vec) || (g.k_contig_b_only && BNM) || (g.f64_only && sizeof(T) != 8)))
+      continue;
     const long tm = (d.M + g.BM - 1) / g.BM, tn = (d.N + g.BN - 1) / g.BN;
     const double pad = (double(tm * g.BM) * double(tn * g.BN)) / (double(d.M) * d.N);
     const long tiles = tm * tn * nbatch;

@@ -52,7 +52,7 @@ def test_gemm_fp64(M, N, K, ta, tb):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
-N_FP64_TILES = 14  # configurations of the generic MFMA engine
+N_FP64_TILES = 17  # configurations of the generic MFMA engine
 N_BF16_TILES = 5   # configurations of the BF16 hi/lo kernel
 
 
@@ -75,10 +75,11 @@ def test_gemm_fp64_every_tile(tile, M, N, K, ta, tb, elementwise):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))) * np.sqrt(K / 1000 + 1)
 
 
+@pytest.mark.parametrize("tile", [3, 15])  # 128 x 128 on 4 and on 8 waves
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("M,N,K,ta,tb", [(2900, 2950, 1000, 0, 0), (2944, 2944, 1024, 0, 1), (3000, 2900, 999, 1, 0),
                                          (2900, 2950, 1000, 1, 1)])
-def test_gemm_split_tail(prec, M, N, K, ta, tb):
+def test_gemm_split_tail(prec, M, N, K, ta, tb, tile):
     """128 x 128 tiles (forced) on more tiles than one full round of the GPU:
     512 tiles run whole and the rest as k-chunks whose partials are reduced
     afterwards (ragged M, N and last k-chunk; all four operand layouts)."""
@@ -88,7 +89,7 @@ def test_gemm_split_tail(prec, M, N, K, ta, tb):
     B = rng.standard_normal((N, K) if tb else (K, N))
     C0 = rng.standard_normal((M, N))
     ref = 0.9 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.2 * C0
-    out = capi.selftest_gemm(A, B, C0, 0.9, -0.2, ta=bool(ta), tb=bool(tb), precision=prec, tile=3)
+    out = capi.selftest_gemm(A, B, C0, 0.9, -0.2, ta=bool(ta), tb=bool(tb), precision=prec, tile=tile)
     tol = 1e-12 if prec == "fp64" else 2e-5
     assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K) * max(1.0, np.abs(ref).max() / np.sqrt(K))
 
