@@ -108,6 +108,19 @@ def test_gemm_lowprec_every_tile(prec, tile, tol, M, N, K):
     assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K)
 
 
+@pytest.mark.parametrize("tile", [1, 2])  # BF16 128 x 128 (552 tiles: 512 whole + 40 split), 256 x 256 (144, all split)
+def test_gemm_bf16_split_tail(tile):
+    capi = _ctx()
+    M, N, K = 2900, 2950, 1000
+    rng = np.random.default_rng(tile + 7)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = A @ B - 0.5 * C0
+    out = capi.selftest_gemm(A, B, C0, 1.0, -0.5, precision="bf16x3", tile=tile)
+    assert np.max(np.abs(out - ref)) <= 5e-5 * np.sqrt(K)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 5e-5)])
 @pytest.mark.parametrize("M,N,K", [(130, 70, 300), (256, 256, 1024), (33, 129, 65)])
 def test_gemm_lowprec(prec, tol, M, N, K):
