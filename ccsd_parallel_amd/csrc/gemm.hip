@@ -82,8 +82,11 @@ struct KArgs {
   T* tail_partial;
 };
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool AK, bool BNM, int WPE, bool VEC>
+// VWT: elements per vector load (2: 16-byte double pairs or 8-byte float
+// pairs, 4: 16-byte float quads), 0 for single-element loads (in units of 2).
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool AK, bool BNM, int WPE, int VWT>
 __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
+  constexpr bool VEC = VWT > 0;
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM / 16;
   constexpr int TN = BN / WN / 16;
@@ -100,7 +103,9 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   constexpr int BSZ = BNM ? BK * LDMB : BN * LDK;
   __shared__ __attribute__((aligned(16))) T As[2][ASZ];
   __shared__ __attribute__((aligned(16))) T Bs[2][BSZ];
-  using pair_t = T __attribute__((ext_vector_type(2)));
+  // load unit: VW consecutive elements, one vector load (VEC) and one LDS store
+  constexpr int VW = VEC ? VWT : 2;
+  using pair_t = T __attribute__((ext_vector_type(VW)));
 
   const GemmDesc<T>& d = p.d;
   // XCD-aware tile id: hardware places linear workgroup x on XCD x % 8, so
@@ -150,29 +155,30 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   const int wy = wave / WN, wx = wave % WN;
   const int wm0 = wy * (BM / WM), wn0 = wx * (BN / WN);
 
-  constexpr int AU = BM * BK / 2;
-  constexpr int BU = BK * BN / 2;
+  static_assert(BK % VW == 0 && BM % VW == 0 && BN % VW == 0, "vector width");
+  constexpr int AU = BM * BK / VW;
+  constexpr int BU = BK * BN / VW;
   constexpr int AIT = (AU + NT - 1) / NT;
   constexpr int BIT = (BU + NT - 1) / NT;
   pair_t ra[AIT], rb[BIT];
 
-  // Unit u of a tile -> (row, k) of its element pair.  A pair runs along the
-  // operand's contiguous index: k for AK / !BNM, m for !AK, n for BNM.
+  // Unit u of a tile -> (row, k) of its VW-element vector.  A vector runs
+  // along the operand's contiguous index: k for AK / !BNM, m for !AK, n for BNM.
   auto a_unit = [](int u, int& m, int& kk) {
-    if (AK) { m = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
-    else    { kk = u / (BM / 2); m = (u % (BM / 2)) * 2; }
+    if (AK) { m = u / (BK / VW); kk = (u % (BK / VW)) * VW; }
+    else    { kk = u / (BM / VW); m = (u % (BM / VW)) * VW; }
   };
   auto b_unit = [](int u, int& n, int& kk) {
-    if (BNM) { kk = u / (BN / 2); n = (u % (BN / 2)) * 2; }
-    else     { n = u / (BK / 2); kk = (u % (BK / 2)) * 2; }
+    if (BNM) { kk = u / (BN / VW); n = (u % (BN / VW)) * VW; }
+    else     { n = u / (BK / VW); kk = (u % (BK / VW)) * VW; }
   };
   // Rows past M and columns past N only feed outputs that the epilogue drops,
   // so their loads are clamped into range rather than masked (a mask would
   // have to wait for the load).  Only k is checked, and only in the ragged
   // last tile: full tiles (`checked` false) load without any test.
-  // VEC: one 2-element load per pair (the host guarantees unit stride along
-  // the pair, even pitches and an aligned base; M or N even when the pair
-  // runs along m or n).
+  // VEC: one load per VW-element vector (the host guarantees unit stride
+  // along the vector, pitches that are multiples of VW and a base aligned to
+  // the vector; M or N a multiple of VW when the vector runs along m or n).
   // Paired loads of an operand whose k stride is not the unit one: each
   // thread's pair addresses at k0 = 0 are computed once, and a tile adds the
   // uniform k0 * pitch (scalar arithmetic) instead of a per-lane 64-bit
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
     for (int it = 0; it < AIT; ++it) {
       int m, kk;
       a_unit(min(it * NT + tid, AU - 1), m, kk);
-      a_base[it] = A + (long)kk * sAk + min(m0 + m, M - 2);
+      a_base[it] = A + (long)kk * sAk + min(m0 + m, M - VW);
     }
   }
   if constexpr (VEC && BNM) {
@@ -192,7 +198,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
     for (int it = 0; it < BIT; ++it) {
       int n, kk;
       b_unit(min(it * NT + tid, BU - 1), n, kk);
-      b_base[it] = B + (long)kk * sBk + min(n0 + n, N - 2);
+      b_base[it] = B + (long)kk * sBk + min(n0 + n, N - VW);
     }
   }
   auto load_tiles = [&](int k0, bool checked) {
@@ -227,16 +233,19 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
       int m, kk;
       a_unit(u, m, kk);
       const int gm = m0 + m, gk = k0 + kk;
-      pair_t v = {T(0), T(0)};
+      pair_t v;
+#pragma unroll
+      for (int e = 0; e < VW; ++e) v[e] = T(0);
       if (AU % NT == 0 || u < AU) {
         if (AK) {
           const T* row = A + (long)min(gm, M - 1) * sAm;
-          if (!checked || gk < kend) v[0] = row[gk * sAk];
-          if (!checked || gk + 1 < kend) v[1] = row[(gk + 1) * sAk];
+#pragma unroll
+          for (int e = 0; e < VW; ++e)
+            if (!checked || gk + e < kend) v[e] = row[(long)(gk + e) * sAk];
         } else if (!checked || gk < kend) {
           const T* kr = A + (long)gk * sAk;
-          v[0] = kr[min(gm, M - 1) * sAm];
-          v[1] = kr[min(gm + 1, M - 1) * sAm];
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[e] = kr[min(gm + e, M - 1) * sAm];
         }
       }
       ra[it] = v;
@@ -247,16 +256,19 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
       int n, kk;
       b_unit(u, n, kk);
       const int gn = n0 + n, gk = k0 + kk;
-      pair_t v = {T(0), T(0)};
+      pair_t v;
+#pragma unroll
+      for (int e = 0; e < VW; ++e) v[e] = T(0);
       if (BU % NT == 0 || u < BU) {
         if (!BNM) {
           const T* col = B + (long)min(gn, N - 1) * sBn;
-          if (!checked || gk < kend) v[0] = col[gk * sBk];
-          if (!checked || gk + 1 < kend) v[1] = col[(gk + 1) * sBk];
+#pragma unroll
+          for (int e = 0; e < VW; ++e)
+            if (!checked || gk + e < kend) v[e] = col[(long)(gk + e) * sBk];
         } else if (!checked || gk < kend) {
           const T* kr = B + (long)gk * sBk;
-          v[0] = kr[min(gn, N - 1) * sBn];
-          v[1] = kr[min(gn + 1, N - 1) * sBn];
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[e] = kr[min(gn + e, N - 1) * sBn];
         }
       }
       rb[it] = v;
@@ -605,32 +617,35 @@ constexpr Cfg kCfgs[] = {
 constexpr int kNumCus = 256;
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, int WPE, bool VEC>
+template <typename T, int BM, int BN, int BK, int WM, int WN, int WPE, int VWT>
 hipError_t launch_layout(const KArgs<T>& ka, bool AK, bool BNM, dim3 grid, hipStream_t s) {
   constexpr int NT = 64 * WM * WN;
   if (AK && BNM)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true, WPE, VWT>), grid, dim3(NT), 0, s, ka);
   else if (AK && !BNM)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false, WPE, VWT>), grid, dim3(NT), 0, s, ka);
   else if (!AK && BNM)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true, WPE, VWT>), grid, dim3(NT), 0, s, ka);
   else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false, WPE, VEC>), grid, dim3(NT), 0, s, ka);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false, WPE, VWT>), grid, dim3(NT), 0, s, ka);
   return hipGetLastError();
 }
 
 template <typename T, int C>
-hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, bool vec, dim3 grid, hipStream_t s) {
+hipError_t launch_cfg(const KArgs<T>& ka, bool AK, bool BNM, int vw, dim3 grid, hipStream_t s) {
   constexpr Cfg g = kCfgs[C];
   // waves-per-SIMD floor handed to the compiler: a CU's 4 SIMDs hold slots x
   // (WM x WN) waves, which must fit 512 registers per SIMD lane between them
   constexpr int wpe = (g.WM * g.WN == 4) ? g.slots : (g.WM * g.WN == 8 && g.slots == 2) ? 4 : 1;
-  if (vec) return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, true>(ka, AK, BNM, grid, s);
-  return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, false>(ka, AK, BNM, grid, s);
+  if constexpr (sizeof(T) == 4) {
+    if (vw == 4) return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, 4>(ka, AK, BNM, grid, s);
+  }
+  if (vw == 2) return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, 2>(ka, AK, BNM, grid, s);
+  return launch_layout<T, g.BM, g.BN, g.BK, g.WM, g.WN, wpe, 0>(ka, AK, BNM, grid, s);
 }
 
 template <typename T>
-hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, bool vec, dim3 grid, hipStream_t s) {
+hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, int vec, dim3 grid, hipStream_t s) {
   switch (c) {
     case 0: return launch_cfg<T, 0>(ka, AK, BNM, vec, grid, s);
     case 1: return launch_cfg<T, 1>(ka, AK, BNM, vec, grid, s);
@@ -653,16 +668,17 @@ hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, bool vec, di
 }
 static_assert(kNumCfgs == 17, "launch_any covers every configuration");
 
-// Paired (2-element vector) loads are legal when each operand has unit stride
-// along its pair index, every other stride and batch stride is even, the base
-// is aligned to a pair, and a pair along m or n cannot straddle M or N.
+// W-element vector loads are legal when each operand has unit stride along
+// its vector index, every other stride and batch stride is a multiple of W,
+// the base is aligned to a vector, and a vector along m or n cannot straddle
+// M or N.
 template <typename T>
-bool pairs_ok(const GemmDesc<T>& d, bool AK, bool BNM) {
-  auto even = [](long x) { return (x & 1) == 0; };
-  auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0; };
-  const bool a = AK ? (d.sAk == 1 && even(d.sAm)) : (d.sAm == 1 && even(d.sAk) && even(d.M));
-  const bool b = BNM ? (d.sBn == 1 && even(d.sBk) && even(d.N)) : (d.sBk == 1 && even(d.sBn));
-  return a && b && even(d.sA1) && even(d.sA2) && even(d.sB1) && even(d.sB2) && aligned(d.A) && aligned(d.B);
+bool vectors_ok(const GemmDesc<T>& d, bool AK, bool BNM, long W) {
+  auto mult = [W](long x) { return x % W == 0; };
+  auto aligned = [W](const void* p) { return (reinterpret_cast<uintptr_t>(p) % (W * sizeof(T))) == 0; };
+  const bool a = AK ? (d.sAk == 1 && mult(d.sAm)) : (d.sAm == 1 && mult(d.sAk) && mult(d.M));
+  const bool b = BNM ? (d.sBn == 1 && mult(d.sBk) && mult(d.N)) : (d.sBk == 1 && mult(d.sBn));
+  return a && b && mult(d.sA1) && mult(d.sA2) && mult(d.sB1) && mult(d.sB2) && aligned(d.A) && aligned(d.B);
 }
 
 }  // namespace
@@ -699,7 +715,11 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   // A is "k-contiguous" unless its m stride is the unit one (and likewise B).
   const bool AK = !(d.sAm == 1 && d.sAk != 1);
   const bool BNM = !(d.sBk == 1 && d.sBn != 1);
-  const bool vec = force_pairs != 0 && pairs_ok(d, AK, BNM);
+  // widest legal vector: float quads (16 bytes), else pairs, else elements
+  const int vw = force_pairs == 0 ? 0
+                 : (sizeof(T) == 4 && vectors_ok(d, AK, BNM, 4)) ? 4
+                 : vectors_ok(d, AK, BNM, 2) ? 2 : 0;
+  const bool vec = vw > 0;
 
   // Tile shape and split-K count with the least modelled time: each of the
   // tiles x splits work units costs its padded flops at the configuration's
@@ -812,7 +832,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   if (ka.tiles_pad <= 0 || splits > 65535) return hipErrorInvalidValue;
   dim3 grid(ka.tiles_pad, nbatch, splits);
   if (tail_split) grid = dim3((unsigned)(8 * (xcd_full + (ka.tail_units + 7) / 8)), 1, 1);
-  hipError_t e = launch_any<T>(best, ka, AK, BNM, vec, grid, stream);
+  hipError_t e = launch_any<T>(best, ka, AK, BNM, vw, grid, stream);
   if (e != hipSuccess) return e;
   if (tail_split) {
     const long n = tail * g.BM * g.BN;
@@ -835,7 +855,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     }
     e = hipGetLastError();
   }
-  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best, vec ? 1 : 0, tail_split};
+  if (info) *info = GemmLaunchInfo{tiles_m, tiles_n, nbatch, splits, best, vw, tail_split};
   return e;
 }
 

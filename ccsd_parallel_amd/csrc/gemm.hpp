@@ -42,7 +42,7 @@ struct GemmWorkspace {
 // Launch statistics of the last call (for tests and the profiler notes).
 struct GemmLaunchInfo {
   int tiles_m = 0, tiles_n = 0, batches = 0, splits = 0, config = 0;
-  int pairs = 0;       // 1 when the operands were loaded as element pairs
+  int pairs = 0;       // elements per vector load (2 or 4), 0 for element loads
   int tail_split = 0;  // k-chunks of the tiles past the last full round (0: none)
 };
 
