@@ -95,7 +95,8 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   // LDK = BK + 2 doubles the 16 rows of one MFMA operand fetch fall in distinct
   // banks.  An m/n-contiguous operand is stored [k][LDM]; the pitch BM + 16
   // puts consecutive k-rows of a fetch in opposite bank halves.  Every global
-  // load brings a pair of elements that are adjacent in LDS too.
+  // load brings a vector of VW elements that are adjacent in LDS too (FP32
+  // pitches: BK + 4 and BM + 32, keeping 16-byte stores aligned).
   constexpr int LDK = BK + (sizeof(T) == 8 ? 2 : 4);
   constexpr int LDMA = BM + (sizeof(T) == 8 ? 16 : 32);
   constexpr int LDMB = BN + (sizeof(T) == 8 ? 16 : 32);
