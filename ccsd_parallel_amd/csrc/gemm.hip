@@ -6,16 +6,20 @@
 //  * A and B tiles are staged through LDS ([m][k] with a padded pitch for a
 //    k-contiguous operand, [k][m] otherwise) so that a wave's operand fetch
 //    (lane l reads row l%16 of k-slice l/16) puts its 16 rows in distinct
-//    banks; LDS is
-//    double-buffered with one barrier per k-tile, and the MFMA operands are
-//    double-buffered in registers (k-step s+1 read during k-step s).
+//    banks; LDS is double-buffered with one barrier per k-tile, and the MFMA
+//    operands are double-buffered in registers (k-step s+1 read during k-step
+//    s).
 //  * Global loads are coalesced along whichever operand stride is unit
 //    (template switches AK / BNM select the loop order); any strides work.
-//    Where the strides allow, each thread loads element pairs with one
-//    16-byte load (VEC); rows/columns past M/N are clamped, not masked, and
-//    only the ragged last k-tile is bounds-checked.
-//  * 4-wave tiles are compiled for two or more resident workgroups per CU, so
-//    one workgroup's barrier wait is covered by another's MFMAs.
+//    Where the strides allow, each thread loads 16-byte vectors (two doubles
+//    or four floats; FP32 falls back to 8-byte pairs); rows/columns past M/N
+//    are clamped, not masked, and only the ragged last k-tile is
+//    bounds-checked.
+//  * Tiles are compiled for two or more resident workgroups per CU (4-wave
+//    tiles at 256 registers, 8-wave FP64 tiles at 128), so one workgroup's
+//    barrier wait is covered by another's MFMAs.
+//  * The tiles past the last full round of the GPU run as k-chunks whose
+//    partials are reduced afterwards (split tail).
 //  * Workgroup ids are remapped so that each XCD (the dispatcher deals
 //    workgroups round-robin over the 8 XCDs, each with its own L2) runs a
 //    contiguous, balanced range of tiles that share operand panels.
