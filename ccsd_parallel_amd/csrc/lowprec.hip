@@ -199,9 +199,13 @@ __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, f
 // of 32.  Each thread moves 16-byte pieces of the interleaved rows (8 per row
 // per k-tile: 4 hi + 4 lo) through registers into double-buffered LDS.  A row
 // keeps its global order (128 bytes: hi0 lo0 hi1 lo1 .. lo3) with the 16-byte
-// chunk index XOR-swizzled by (row >> 1) & 7, and no padding: 16 lanes store
-// two whole rows (256 contiguous bytes), and the 16 rows of one MFMA operand
-// fetch land in distinct banks.  The next k-tile's loads are issued
+// chunk index XOR-swizzled by (row & 7) ^ ((row >> 3) & 1), and no padding: 16
+// lanes store two whole rows (256 contiguous bytes), and the 16 rows of one
+// MFMA operand fetch (rows 16i .. 16i + 15, one chunk each) read without bank
+// conflicts: a lane group's 8 rows 0..7 and 8..15 take distinct chunks, and
+// rows of equal parity in the two halves take chunks of opposite parity.  (The
+// swizzle (row >> 1) & 7 measured two-way conflicted reads on gfx950, 8 LDS
+// cycles per 16-byte wave read instead of 4.)  The next k-tile's loads are issued
 // unconditionally before the current tile's MFMAs (the last iteration reloads
 // its own tile into the idle buffer) so they stay in flight across the
 // compute; rows past M / N are clamped to the last row (they only feed
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * RP];
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * RP];
   // LDS slot of 16-byte chunk q of tile row m
-  auto slot = [](int m, int q) { return m * RP + ((q ^ ((m >> 1) & 7)) << 3); };
+  auto slot = [](int m, int q) { return m * RP + ((q ^ (m & 7) ^ ((m >> 3) & 1)) << 3); };
 
   int tile, kt0 = 0, kt1 = p.Kp / 32;  // k-tile range of this unit
   float* tail_out = nullptr;           // this unit's FP32 partial (split tail tiles)
@@ -299,8 +303,8 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
       if (it < CB - 1 || lb) *(u32x4*)&sB[buf][sb[it]] = rb[it];
   };
   // operand fetch: lane l reads tile row 16i + (l & 15) (a multiple of 16 plus
-  // l & 15, so its swizzle is ((l & 15) >> 1) & 7), k-slice l >> 4
-  const int swz = (lane >> 1) & 7, ks = lane >> 4;
+  // l & 15, so its swizzle is (l & 7) ^ ((l >> 3) & 1)), k-slice l >> 4
+  const int swz = (lane & 7) ^ ((lane >> 3) & 1), ks = lane >> 4;
   const int oh = ((2 * ks) ^ swz) << 3, ol = ((2 * ks + 1) ^ swz) << 3;
   struct AFrag { bf16x8_t h[TM], l[TM]; };
   auto read_a = [&](int buf, AFrag& f) {
