@@ -97,13 +97,17 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
   static_assert(TM >= 1 && TN >= 1 && TM * WM * 16 == BM && TN * WN * 16 == BN, "wave tile");
   // LDS layouts.  A k-contiguous operand stays k-contiguous ([m][LDK]): with
   // LDK = BK + 2 doubles the 16 rows of one MFMA operand fetch fall in distinct
-  // banks.  An m/n-contiguous operand is stored [k][LDM]; the pitch BM + 16
-  // puts consecutive k-rows of a fetch in opposite bank halves.  Every global
-  // load brings a vector of VW elements that are adjacent in LDS too (FP32
-  // pitches: BK + 4 and BM + 32, keeping 16-byte stores aligned).
+  // banks.  An m/n-contiguous operand is stored [k][LDM] with LDM = 16 mod 32
+  // elements (BM + 16 or BM + 32): the k-rows of one fetch then start 16 or 32
+  // banks apart, which gfx950 serves without conflicts for both 8-byte and
+  // 4-byte reads (measured: pitches 80 and 144 conflict-free; 64, 96, 128 and
+  // 160 two-way conflicted).  Every global load brings a vector of VW
+  // elements that are adjacent in LDS too (FP32 pitches are multiples of 4,
+  // keeping 16-byte stores aligned; FP32 LDK = BK + 4).
   constexpr int LDK = BK + (sizeof(T) == 8 ? 2 : 4);
-  constexpr int LDMA = BM + (sizeof(T) == 8 ? 16 : 32);
-  constexpr int LDMB = BN + (sizeof(T) == 8 ? 16 : 32);
+  constexpr int LDMA = BM + ((BM & 31) ? 32 : 16);
+  constexpr int LDMB = BN + ((BN & 31) ? 32 : 16);
+  static_assert(LDMA % 32 == 16 && LDMB % 32 == 16, "m/n pitch");
   constexpr int ASZ = AK ? BM * LDK : BK * LDMA;
   constexpr int BSZ = BNM ? BK * LDMB : BN * LDK;
   __shared__ __attribute__((aligned(16))) T As[2][ASZ];
