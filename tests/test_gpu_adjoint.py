@@ -27,7 +27,9 @@ def _prob(O, V, seed, noncanon=False):
     return prob
 
 
-@pytest.mark.parametrize("O,V,seed,noncanon", [(4, 8, 31, False), (6, 12, 32, True), (8, 16, 33, False)])
+# (3, 5) and (2, 3): odd spin-orbital counts and single-tile edges
+@pytest.mark.parametrize("O,V,seed,noncanon", [(3, 5, 35, True), (2, 3, 36, False), (4, 8, 31, False), (6, 12, 32, True),
+                                               (8, 16, 33, False)])
 def test_backward_matches_cpu_adjoint(O, V, seed, noncanon):
     capi = _capi()
     prob = _prob(O, V, seed, noncanon)

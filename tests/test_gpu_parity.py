@@ -136,8 +136,11 @@ def test_gemm_lowprec(prec, tol, M, N, K):
 
 
 # (12, 40) and (14, 60): enough packed pairs that the split-K and batch
-# reductions (F_mi, F_ae, the T1 tau.<ma||ef> term) take the many-slice path
-@pytest.mark.parametrize("O,V,seed", [(4, 8, 11), (6, 10, 12), (8, 20, 13), (10, 26, 14), (12, 40, 15), (14, 60, 16)])
+# reductions (F_mi, F_ae, the T1 tau.<ma||ef> term) take the many-slice path.
+# (2, 2): one occupied and one virtual pair (every product has a unit edge);
+# (3, 5), (5, 3), (2, 3): odd spin-orbital counts, and V < O.
+@pytest.mark.parametrize("O,V,seed", [(2, 2, 17), (3, 5, 18), (5, 3, 19), (2, 3, 20), (4, 8, 11), (6, 10, 12),
+                                      (8, 20, 13), (10, 26, 14), (12, 40, 15), (14, 60, 16)])
 def test_residual_parity(O, V, seed):
     capi = _ctx()
     prob = synth.make_problem(O, V, seed, scale=0.05)
@@ -249,14 +252,14 @@ def test_low_precision_fixed_point_is_near_fp64(prec, floor):
     assert abs(tr[-1] - e64) <= floor * max(1.0, abs(e64))
 
 
+@pytest.mark.parametrize("O,V,seed", [(8, 36, 77), (3, 5, 78)])  # (3, 5): odd counts, every tile mostly clamped rows
 @pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 5e-4)])
-def test_low_precision_update_matches_fp64_update(prec, tol):
+def test_low_precision_update_matches_fp64_update(prec, tol, O, V, seed):
     """One amplitude update from the same amplitudes (T1 nonzero everywhere) in
     FP64 and in low precision.  Every contraction of the low-precision residual
     -- the BF16/FP32 dominant GEMMs and the FP32 secondary ones -- enters the
     new amplitudes, which must agree to the rounding level of the mode."""
     capi = _ctx()
-    O, V, seed = 8, 36, 77
     sc = norm_scale(O, V)
     rng = np.random.default_rng(5)
     with capi.Context.synthetic(O, V, seed, sc) as ctx:
