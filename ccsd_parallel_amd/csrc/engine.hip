@@ -396,10 +396,13 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   // -- packed residual: <ab||ij> + pp ladder + hh ladder
   k_copy(c->Rp, c->oovv, po * pv, s);
   if (time_it) HIPTRY(hipEventRecord(c->ev[5], s));
+  // pp ladder Rp[ij][ab] += sum_{e<f} tau[ij][ef] <ab||ef> (Fill_t2_new :2093):
+  // row ab of the packed vvvv block is k-contiguous, so B is read as rows
+  // (which also admits the 8-wave 112 x 128 tile; an n-contiguous B does not)
   if (lowp) {
     TRY(lowprec_ladder(c, prec));
   } else {
-    TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, pv, 1, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0)));
+    TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, 1, pv, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0)));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[6], s));
   {

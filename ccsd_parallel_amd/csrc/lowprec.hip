@@ -730,7 +730,8 @@ int lowprec_wmbej(Ctx* c, int prec) {
 int lowprec_ladder(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
-  // <ef||ab> = <ab||ef>: the packed vvvv block is symmetric, rows are k-contiguous B.
+  // Rp[ij][ab] += sum_{e<f} tau[ij][ef] <ab||ef>: row ab of the packed vvvv
+  // block is the k-contiguous B row of output column ab.
   return nt_product(c, prec, L->tau32, L->tau_b, L->ld_pv, L->vvvv32, L->vvvv_b, L->ld_pv, (int)d.po, (int)d.pv,
                     (int)d.pv, c->Rp, d.pv, 1.0, 1.0);
 }

@@ -171,6 +171,53 @@ def test_residual_parity(O, V, seed):
         assert abs(ctx.energy(t1, packed(t2, O, V)) - ccsd_ref.energy(I, t1, t2)) <= 1e-13 * max(1, abs(ccsd_ref.energy(I, t1, t2)))
 
 
+def _asymmetric_vvvv_problem(O, V, seed):
+    prob = synth.make_problem(O, V, seed, scale=0.05)
+    rng = np.random.default_rng(seed)
+    q = rng.standard_normal(prob["vvvv"].shape)
+    return dict(prob, vvvv=prob["vvvv"] + 0.01 * (q - q.T))
+
+
+def test_residual_parity_asymmetric_vvvv():
+    """The pp ladder contracts tau_ij^ef with <ab||ef>: row ab, column ef of the
+    packed block, as Fill_t2_new does (ccsd_memory_optimized.cpp:2093).  A
+    block that is not symmetric under pair exchange must still give the
+    oracle's residual (contracting it the other way round moves the ladder
+    term by about half its size here)."""
+    capi = _ctx()
+    O, V, seed = 6, 12, 21
+    prob = _asymmetric_vvvv_problem(O, V, seed)
+    I = ccsd_ref.Ints.from_problem(prob)
+    t1, t2 = random_amplitudes(O, V, seed)
+    R1, R2 = ccsd_ref.rhs(I, t1, t2)[:2]
+    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+        r1, r2 = ctx.residual(t1, packed(t2, O, V))
+    scale = max(np.abs(R2).max(), np.abs(R1).max())
+    assert np.max(np.abs(r2 - packed(R2, O, V))) <= 1e-12 * scale
+    assert np.max(np.abs(r1 - R1)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 5e-4)])
+def test_low_precision_update_asymmetric_vvvv(prec, tol):
+    """The low-precision ladder contracts the packed block the same way round
+    as the FP64 one: one update from the same amplitudes agrees to the
+    rounding level of the mode on a block that is not pair-exchange symmetric."""
+    capi = _ctx()
+    O, V, seed = 6, 12, 22
+    prob = _asymmetric_vvvv_problem(O, V, seed)
+    t1, t2 = random_amplitudes(O, V, seed, amp=0.02)
+    out = {}
+    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+        for p in ("fp64", prec):
+            ctx.set_amplitudes(t1, packed(t2, O, V))
+            ctx.iterate(1, p)
+            out[p] = ctx.get_amplitudes()
+    (a1, a2), (b1, b2) = out["fp64"], out[prec]
+    e2 = np.max(np.abs(b2 - a2)) / np.max(np.abs(a2))
+    e1 = np.max(np.abs(b1 - a1)) / np.max(np.abs(a1))
+    assert e2 <= tol and e1 <= tol, (e1, e2)
+
+
 def test_synthetic_generator_matches_host():
     """Device-generated integrals equal the host generator (up to libm rounding)."""
     capi = _ctx()
