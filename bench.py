@@ -273,7 +273,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": ladder, "peak": PEAK[prec], "unit": "TFLOP/s",
                      "frac": (ladder / PEAK[prec]) if ladder else None,
                      "traffic": ladder_traffic(O, V, prec), "traffic_unit": "bytes/launch",
-                     "kernel": "pp ladder GEMM tau[P2(O) x P2(V)] x <ef||ab>[P2(V) x P2(V)]",
+                     "kernel": "pp ladder GEMM tau[P2(O) x P2(V)] x <ab||ef>^T (packed vvvv rows ab, k = ef)",
                      "algorithmic_flops": fl["pp_ladder"]},
     }
 
