@@ -17,10 +17,11 @@
  *     ovov [O*V][O*V]       <ia||jb>, row i*V+a
  *     ovvv [O*V][P2(V)]     <ia||bc>, b<c
  *     vvvv [P2(V)][P2(V)]   <ab||cd>, a<b, c<d
- *   The amplitude equations read each block as stored (the pp ladder sums
- *   tau_ij^cd <ab||cd> over row ab of vvvv); they assume no symmetry under
- *   pair exchange.  The gradients of ccsd_backward are symmetrised over it,
- *   which presumes real orbitals, as the reference does.
+ *   Orbitals are real, as in the reference: ovov must satisfy
+ *   <ia||jb> = <jb||ia> (W_mbej and the T1 ring term use it).  oooo and
+ *   vvvv are read as stored (the pp ladder sums tau_ij^cd <ab||cd> over row
+ *   ab of vvvv), and the gradients of ccsd_backward are symmetrised over
+ *   pair exchange.
  *   fock is the full N x N Fock matrix (row-major, occupied first).
  *   Amplitudes: t1[i*V+a]; t2 packed [P2(O)][P2(V)] holding t_ij^ab, i<j, a<b.
  *   Return value 0 = success, negative = error (ccsd_strerror).  Host buffers

@@ -197,6 +197,33 @@ def test_residual_parity_asymmetric_vvvv():
     assert np.max(np.abs(r1 - R1)) <= 1e-12 * scale
 
 
+def test_residual_parity_asymmetric_oooo_vvvv():
+    """oooo and vvvv are read as stored, <row||column>: with both made
+    asymmetric under pair exchange (transposing either moves the residual by
+    12-21% here) the residual and W_mnij still equal the oracle's.  ovov is
+    the exception: W_mbej and the T1 ring term take <ia||jb> = <jb||ia>
+    (real orbitals, as the reference)."""
+    capi = _ctx()
+    O, V, seed = 5, 9, 23
+    prob = synth.make_problem(O, V, seed, scale=0.05)
+    rng = np.random.default_rng(seed)
+    for blk in ("oooo", "vvvv"):
+        q = rng.standard_normal(prob[blk].shape)
+        prob[blk] = prob[blk] + 0.01 * (q - q.T)
+    I = ccsd_ref.Ints.from_problem(prob)
+    t1, t2 = random_amplitudes(O, V, seed)
+    R1, R2, _, _, X = ccsd_ref.rhs(I, t1, t2, parts=True)
+    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+        r1, r2 = ctx.residual(t1, packed(t2, O, V))
+        for name, dev, ref in (("W1", ctx.debug("W1"), synth.pack_pairs(synth.pack_pairs(X["W1"], O, 0), O, 1)),
+                               ("WT", ctx.debug("WT"), X["W4"].transpose(3, 1, 0, 2))):
+            err = np.max(np.abs(dev - ref.ravel()))
+            assert err <= 1e-12 * max(1.0, np.abs(ref).max()), (name, err)
+    scale = max(np.abs(R2).max(), np.abs(R1).max())
+    assert np.max(np.abs(r2 - packed(R2, O, V))) <= 1e-12 * scale
+    assert np.max(np.abs(r1 - R1)) <= 1e-12 * scale
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 5e-4)])
 def test_low_precision_update_asymmetric_vvvv(prec, tol):
     """The low-precision ladder contracts the packed block the same way round
