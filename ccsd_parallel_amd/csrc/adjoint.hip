@@ -15,7 +15,8 @@
 //  and streamed to the host (opt.recompute_v4).
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
-//    until max|dLambda| < lambda_switch_factor * tol, then at least
+//    until max|dLambda| < lambda_switch_factor * tol (or until it stalls:
+//    no new smallest max|dLambda| in kLowpStall iterations), then at least
 //    final_fp64_sweeps FP64 iterations -- the forward's schedule, with the
 //    DIIS subspace kept across the switch.  The gradient pass is always FP64.
 // The reference has no adjoint; its forward algebra is restated in engine.hip.
@@ -632,6 +633,8 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   double delta = 1e300;
   const double tol = 1e-10;
   bool conv = false, lowp = c->opt.precision == CCSD_PREC_BF16X3;
+  double best_low = HUGE_VAL;  // smallest max|dLambda| of the BF16x3 phase
+  int since_best = 0;
   if (lowp) {
     TRY(lowprec_adj_setup(c));
     fp64_after = 0;
@@ -674,7 +677,14 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
       // 15 + 5 iterations against 15 + 7 with a reset), since the tight
       // max|dLambda| < 1e-10 test on FP64 updates pins the FP64 fixed point
       const double f = c->opt.lambda_switch_factor > 0 ? c->opt.lambda_switch_factor : c->opt.switch_factor;
-      if (delta < std::max(f * tol, tol)) lowp = false;
+      if (delta < best_low) {
+        best_low = delta;
+        since_best = 0;
+      } else {
+        ++since_best;
+      }
+      // also switch once the BF16x3 phase stalls at its rounding floor
+      if (delta < std::max(f * tol, tol) || since_best >= kLowpStall) lowp = false;
       continue;
     }
     if (delta < tol && fp64_after >= c->opt.final_fp64_sweeps) { conv = true; break; }

@@ -10,6 +10,13 @@
 
 namespace ccsd {
 
+// The mixed Lambda schedule switches to FP64 when the BF16x3 iteration has
+// set no new smallest max|dLambda| in this many iterations: it then sits at
+// its rounding floor, above the switch threshold.  (The forward needs no such
+// test: the low-precision map settles on its own fixed point, where |dE|
+// falls below any threshold.)
+constexpr int kLowpStall = 4;
+
 struct LowPrec;   // FP32 / BF16 operand copies for the mixed-precision GEMMs
 struct AdjState;  // adjoint (Lambda) buffers, allocated by ccsd_backward
 

@@ -72,7 +72,9 @@ typedef struct ccsd_options {
   double switch_factor;     /* mixed modes: switch once |dE| < switch_factor * tol (default 100: the FP64
                                sweeps then take |dE| down the last two decades) */
   double lambda_switch_factor; /* backward, BF16X3: switch the Lambda iterations to FP64 once
-                                  max|dLambda| < this * 1e-10 (0 = switch_factor) */
+                                  max|dLambda| < this * 1e-10 (0 = switch_factor), or once
+                                  max|dLambda| has set no new minimum in 4 BF16x3 iterations
+                                  (its rounding floor) */
 } ccsd_options;
 
 typedef struct ccsd_problem {

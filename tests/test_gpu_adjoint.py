@@ -155,3 +155,25 @@ def test_backward_mixed_precision_matches_fp64():
     den = np.sqrt(sum(float(np.sum(g64[k] ** 2)) for k in KEYS))
     assert num / den <= 1e-6, num / den
     assert abs(rmx["e_corr"] - r64["e_corr"]) <= 1e-8
+
+
+def test_backward_mixed_switches_at_a_stall():
+    """lambda_switch_factor 1e-6 puts the Lambda switch (1e-10) below the
+    BF16x3 rounding floor: the Lambda iteration switches to FP64 once
+    max|dLambda| has set no new minimum in 4 iterations, and the gradient
+    still matches the FP64 one."""
+    capi = _capi()
+    O, V, seed = 8, 24, 37
+    sc = 1.0 / np.sqrt(O * synth.p2(V))
+    out = {}
+    for prec, opts in (("fp64", {}), ("bf16x3", {"lambda_switch_factor": 1e-6})):
+        with capi.Context.synthetic(O, V, seed, sc, precision=prec, diis_dim=8, **opts) as ctx:
+            ctx.solve(1e-10, 200)
+            g, info = ctx.backward()
+        out[prec] = (g, info)
+    (g64, _), (gmx, imx) = out["fp64"], out["bf16x3"]
+    assert imx["converged"] == 1 and imx["lambda_lowp_iterations"] > 0
+    assert imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 3
+    num = np.sqrt(sum(float(np.sum((gmx[k] - g64[k]) ** 2)) for k in KEYS))
+    den = np.sqrt(sum(float(np.sum(g64[k] ** 2)) for k in KEYS))
+    assert num / den <= 1e-6, num / den
