@@ -278,18 +278,24 @@ def main():
     }
 
     if not args.no_converge:
+        # FP64 oracle for the error statement: the FP64 GPU path converged to 1e-11
+        c = capi.Context.synthetic(O, V, seed, scale, precision="fp64", diis_dim=8)
+        e_tight = c.solve(1e-11, 100, allow_unconverged=True)["e_corr"]
+        t1_tight, t2_tight = c.get_amplitudes()
+        c.close()
         conv = {}
         for p in ("fp64", "fp32", "bf16x3"):
             c = capi.Context.synthetic(O, V, seed, scale, precision=p, diis_dim=8, final_fp64_sweeps=3)
             r = c.solve(1e-7, 100, allow_unconverged=True)
             conv[p] = {k: r[k] for k in ("e_corr", "iterations", "converged", "seconds_total", "lowp_iterations",
                                          "fp64_iterations")}
+            t1, t2 = c.get_amplitudes()
             c.close()
-        c = capi.Context.synthetic(O, V, seed, scale, precision="fp64", diis_dim=8)
-        e_tight = c.solve(1e-11, 100, allow_unconverged=True)["e_corr"]
-        c.close()
-        for p in conv:
             conv[p]["abs_dE_vs_fp64_1e-11"] = abs(conv[p]["e_corr"] - e_tight)
+            conv[p]["max_dT2_vs_fp64_1e-11"] = float(np.abs(t2 - t2_tight).max())
+            conv[p]["max_dT1_vs_fp64_1e-11"] = float(np.abs(t1 - t1_tight).max())
+            del t1, t2
+        del t1_tight, t2_tight
         for p in ("fp32", "bf16x3"):
             conv[p]["abs_dE_vs_fp64"] = abs(conv[p]["e_corr"] - conv["fp64"]["e_corr"])
             conv[p]["speedup_vs_fp64"] = conv["fp64"]["seconds_total"] / conv[p]["seconds_total"]

@@ -224,6 +224,20 @@ class Context:
             raise CCSDError(rc, "ccsd_solve")
         return r.as_dict()
 
+    def solve_status(self, tol=1e-10, max_iter=1000):
+        """(status code, result dict) of ccsd_solve without raising; the result
+        is filled on CCSD_ERR_NOT_CONVERGED and on a non-finite iteration."""
+        r = Result()
+        rc = self.lib.ccsd_solve(self.h, tol, max_iter, C.byref(r))
+        return rc, r.as_dict()
+
+    def backward_status(self):
+        """Status code of ccsd_backward into throw-away arrays (no raising)."""
+        g = {k: np.zeros(sh) for k, sh in self.gradient_shapes().items()}
+        info = GradInfo()
+        return self.lib.ccsd_backward(self.h, *[_ptr(g[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv",
+                                                                      "vvvv")], C.byref(info))
+
     def iterate(self, n, precision="fp64"):
         e = np.zeros(max(n, 1))
         _check(self.lib.ccsd_iterate(self.h, n, PRECISIONS.get(precision, precision), _ptr(e)), "ccsd_iterate")

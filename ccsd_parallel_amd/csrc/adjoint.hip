@@ -11,8 +11,9 @@
 //    cotangents switched on -- the unrelaxed one- and two-particle density
 //    contractions -- plus dE/dtheta and the denominator term.
 //  The four-virtual block gradient (P2(V) x P2(V), 16 GB at O=40 V=300) is
-//  never held whole in HBM: it is recomputed in row slabs from tau and Lambda
-//  and streamed to the host (opt.recompute_v4).
+//  by default never held whole in HBM: it is recomputed in row slabs from tau
+//  and Lambda and streamed to the host (opt.recompute_v4 = 1; with 0 the whole
+//  block is formed in HBM in one pass).
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
 //    until max|dLambda| < lambda_switch_factor * tol (or until it stalls:
@@ -766,15 +767,32 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
     return CCSD_ERR_HIP;
   }
 
-  // four-virtual block: dE/d<ef||ab> = 1/2 (tau^T lam2 + lam2^T tau), in row slabs
-  // computed into two alternating device buffers and streamed to the host
+  // four-virtual block: dE/d<ef||ab> = 1/2 (tau^T lam2 + lam2^T tau).  With
+  // recompute_v4 (default) it is recomputed in 512 MB row slabs, computed into
+  // two alternating device buffers and streamed to the host; without, the
+  // whole P2(V)^2 block is formed in HBM in one pass (CCSD_ERR_BUDGET when it
+  // does not fit in the free device memory)
   {
     const double* lam2 = A->lam + ov;
-    const long slab_rows = std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv));
-    for (int b = 0; b < 2; ++b)
-      if (hipMalloc(&slab[b], (size_t)slab_rows * pv * 8) != hipSuccess) {
+    const long slab_rows =
+        c->opt.recompute_v4 ? std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv)) : pv;
+    const int nbuf = slab_rows >= pv ? 1 : 2;
+    if (!c->opt.recompute_v4) {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
         teardown();
         return CCSD_ERR_HIP;
+      }
+      if ((size_t)pv * pv * 8 > fr) {
+        c->last_error = "four-virtual gradient block does not fit; set recompute_v4";
+        teardown();
+        return CCSD_ERR_BUDGET;
+      }
+    }
+    for (int b = 0; b < nbuf; ++b)
+      if (hipMalloc(&slab[b], (size_t)slab_rows * pv * 8) != hipSuccess) {
+        teardown();
+        return c->opt.recompute_v4 ? CCSD_ERR_HIP : CCSD_ERR_BUDGET;
       }
     long k = 0;
     for (long r0 = 0; r0 < pv && rc == CCSD_OK; r0 += slab_rows, ++k) {

@@ -20,19 +20,6 @@ namespace {
 
 long p2(long n) { return n * (n - 1) / 2; }
 
-int frozen_core_orbitals(const vec& Z) {
-  int n = 0;
-  for (int i = 0; i < Z.n_elem(); ++i) {
-    int z = (int)std::lround(Z(i));
-    if (z > 54) n += 27;
-    else if (z > 36) n += 18;
-    else if (z > 18) n += 9;
-    else if (z > 10) n += 5;
-    else if (z > 2) n += 1;
-  }
-  return n;
-}
-
 }  // namespace
 
 CCSD_Memory_optimized::CCSD_Memory_optimized(int n_N, vec zz, mat rr, std::string B_s, int n_Elec, int ran, int siz,
@@ -49,9 +36,26 @@ CCSD_Memory_optimized::CCSD_Memory_optimized(int n_spatial, int n_occ_spatial, c
 
 CCSD_Memory_optimized::~CCSD_Memory_optimized() = default;
 
+// Freeze_How_Meny_Orbitals (ccsd_memory_optimized.cpp:5203-5231): with
+// freeze_core, one spatial orbital per nucleus with 2 < Z < 11 (Li..Ne) and
+// none for H, He or heavier atoms.  The reference runs its loop to
+// n_Electrons and so reads Z past its last nucleus (unchecked, ARMA_NO_DEBUG);
+// this loop stops at the nuclei.
+int CCSD_Memory_optimized::Freeze_How_Meny_Orbitals() const {
+  int val = 0;
+  if (!freeze_core) return 0;
+  for (int i = 0; i < Z.n_elem(); ++i) {
+    const long z = std::lround(Z(i));
+    if (z > 2 && z < 11) val += 1;
+  }
+  return val;
+}
+
 // Inputs from the HF object (:49, :50, :68, :74, :2987): energy, size, MO
-// coefficients, orbital energies and the AO integral slices (p q | j l), j >= l,
-// unfolded into the full n^4 array.  The transform itself runs on the GPU.
+// coefficients, orbital energies and the AO integral slices
+// Return_Field_Q(j, l)(p, q) = (j l | p q), j >= l -- the reference object's
+// layout after the re-key at the end of its SCF (hartree_fock_solver.cpp:1045-1210)
+// -- unfolded into the full n^4 array.  The transform itself runs on the GPU.
 int CCSD_Memory_optimized::gather_from_provider(double toler) {
   e_hf_ = HartFock->get_Energy(toler, 0);
   const int n = HartFock->ReturnMatrixSize();
@@ -95,7 +99,7 @@ double CCSD_Memory_optimized::CCSD(double toler, bool print_stuff) {
   if (have_opt_) o = opt_;
   else ccsd_default_options(&o);
   o.verbose = print_stuff ? 1 : 0;
-  const int nf = freeze_core ? frozen_core_orbitals(Z) : 0;
+  const int nf = Freeze_How_Meny_Orbitals();
   const int O = 2 * (nocc_ - nf), V = 2 * (n_ - nocc_);
   if (print_stuff) {
     printf("Number of Electrons for CCSD: %d\n", O);

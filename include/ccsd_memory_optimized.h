@@ -61,7 +61,14 @@ class Hartree_Fock_Solver {
   virtual ccsd_compat::mat ReturnC() = 0;
   // Orbital energies, ascending, n entries (:74; a column vector in a mat).
   virtual ccsd_compat::mat return_eigval_F() = 0;
-  // AO two-electron slice for j >= l: M(p, q) = (p q | j l), chemists' notation (:2987).
+  // AO two-electron slice for j >= l: M(p, q) = (j l | p q) = (p q | j l),
+  // chemists' notation (:2987).  This is the slice the reference's object
+  // returns after get_Energy: the SCF fills field_Q(o1,o3)(o2,o4) =
+  // (o1 o2|o3 o4) (hartree_fock_solver.cpp:798-802, used so by the Fock build
+  // at :528), then re-keys it before returning (:1045-1210:
+  // temp_Q(o1,o2)(o3,o4) = field_Q(o1,o3)(o2,o4); field_Q = temp_Q), and
+  // Prepear_AOs contracts the slice over (p, q) and fills the result
+  // symmetrically in (j, l) (ccsd_memory_optimized.cpp:2987-2991, :3025-3026).
   virtual ccsd_compat::mat Return_Field_Q(int j, int l) = 0;
 };
 
@@ -93,6 +100,9 @@ class CCSD_Memory_optimized {
   // int32 O, V; double t1[O*V]; double t2[P2(O)*P2(V)] (the C ABI's layouts).
   void set_checkpoint(const std::string& path) { checkpoint_ = path; }
   bool restarted() const { return restarted_; }
+  // Spatial orbitals left out of the correlation when freeze_core is set
+  // (ccsd_memory_optimized.cpp:5203-5231): one per nucleus with 2 < Z < 11.
+  int Freeze_How_Meny_Orbitals() const;
   int nuclei() const { return n_Nuclei; }
   int mpi_rank() const { return rank; }  // kept for the reference's signature; one process here
   int mpi_size() const { return size; }

@@ -1,4 +1,4 @@
-"""Build recipe for the oracle's C++ restatement (``oracle/ccsd_cpu.cpp``).
+"""Build recipe and binding for the oracle's C++ restatement (``oracle/ccsd_cpu.cpp``).
 
 Test and baseline infrastructure only: ``bench.py``'s CPU baseline and the CPU
 tests load ``oracle/lib/libccsd_cpu.so``; the product never does.
@@ -35,6 +35,8 @@ def load():
     lib.cpu_dot_rows.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_int]
     lib.cpu_dot_rows.restype = C.c_int
     lib.cpu_max_threads.restype = C.c_int
+    lib.cpu_ccsd_residual.argtypes = [C.c_int, C.c_int] + [dp] * 11 + [C.c_int]
+    lib.cpu_ccsd_residual.restype = C.c_int
     return lib
 
 
@@ -51,6 +53,78 @@ def dot_rows(A, B, threads=0):
     if rc != 0:
         raise RuntimeError("cpu_dot_rows failed")
     return out
+
+
+def residual(prob, t1, t2p, threads=0):
+    """R1 (O x V) and packed R2 (P2(O) x P2(V)) of the C++ packed FP64 residual
+    (cpu_ccsd_residual) for a problem dict in the packed layouts of
+    ``oracle.synth`` (keys f, oooo, ooov, oovv, ovov, ovvv, vvvv)."""
+    import ctypes as C
+
+    import numpy as np
+    O, V = prob["O"], prob["V"]
+    po, pv = O * (O - 1) // 2, V * (V - 1) // 2
+    arr = {k: np.ascontiguousarray(prob[k], dtype=np.float64)
+           for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")}
+    t1 = np.ascontiguousarray(t1, dtype=np.float64)
+    t2p = np.ascontiguousarray(t2p, dtype=np.float64)
+    if t1.size != O * V or t2p.size != po * pv:
+        raise ValueError("amplitude shapes")
+    r1 = np.zeros((O, V))
+    r2 = np.zeros((po, pv))
+    ptr = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rc = load().cpu_ccsd_residual(O, V, *[ptr(arr[k]) for k in ("f", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")],
+                                  ptr(t1), ptr(t2p), ptr(r1), ptr(r2), int(threads))
+    if rc != 0:
+        raise RuntimeError("cpu_ccsd_residual failed")
+    return r1, r2
+
+
+def solve(prob, toler=1e-10, max_iter=100, diis_dim=0, threads=0):
+    """The reference driver loop (ccsd_memory_optimized.cpp:473-557: MP2 guess,
+    T <- R(T)/D, stop at |dE| < toler) with optional DIIS
+    (``oracle.ccsd_ref.DIIS``), on the packed C++ residual: the same iteration
+    as ``ccsd_ref.solve`` at sizes the dense oracle cannot hold.  Returns
+    (E, t1, packed t2, info) with info['energies'] (MP2 guess first)."""
+    import numpy as np
+
+    from . import ccsd_ref, synth
+    O, V = prob["O"], prob["V"]
+    f = prob["f"]
+    eo, ev = np.diag(f)[:O], np.diag(f)[O:]
+    ip, iq = synth.pair_list(O)
+    ap, aq = synth.pair_list(V)
+    D2 = (eo[ip] + eo[iq])[:, None] - (ev[ap] + ev[aq])[None, :]
+    D1 = eo[:, None] - ev[None, :]
+    oovv = prob["oovv"]
+    fov = f[:O, O:]
+
+    def energy(t1, t2):
+        with np.errstate(over="ignore", invalid="ignore"):  # a divergent run reports a non-finite E
+            tau = t2 + (t1[ip][:, ap] * t1[iq][:, aq] - t1[ip][:, aq] * t1[iq][:, ap])
+            return float(np.sum(fov * t1) + np.sum(oovv * tau))
+
+    t1, t2 = np.zeros((O, V)), oovv / D2
+    E = energy(t1, t2)
+    energies = [E]
+    diis = ccsd_ref.DIIS(diis_dim) if diis_dim and diis_dim >= 2 else None
+    converged = False
+    for _ in range(max_iter):
+        r1, r2 = residual(prob, t1, t2, threads)
+        n1, n2 = r1 / D1, r2 / D2
+        if diis is not None:
+            x = diis.push_and_extrapolate(np.concatenate([n1.ravel(), n2.ravel()]),
+                                          np.concatenate([t1.ravel(), t2.ravel()]))
+            n1, n2 = x[:O * V].reshape(O, V), x[O * V:].reshape(n2.shape)
+        t1, t2 = n1, n2
+        E_old, E = E, energy(t1, t2)
+        energies.append(E)
+        if not np.isfinite(E):
+            break
+        if abs(E - E_old) < toler:
+            converged = True
+            break
+    return E, t1, t2, dict(energies=energies, iterations=len(energies) - 1, converged=converged)
 
 
 if __name__ == "__main__":

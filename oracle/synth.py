@@ -123,8 +123,10 @@ _BLOCK_KINDS = {
 }
 
 
-def packed_block(name, O, V, seed, scale=0.02, rows=None, dtype=np.float64):
-    """Generate one packed block (optionally only a row range ``rows=(r0, r1)``)."""
+def packed_block(name, O, V, seed, scale=0.02, rows=None, dtype=np.float64, workers=1):
+    """Generate one packed block (optionally only a row range ``rows=(r0, r1)``);
+    ``workers`` > 1 fills row chunks on that many threads (numpy releases the
+    GIL inside the element-wise kernels)."""
     sets = _index_sets(O, V)
     rk, ck = _BLOCK_KINDS[name]
     rp, rq = sets[rk]
@@ -133,19 +135,29 @@ def packed_block(name, O, V, seed, scale=0.02, rows=None, dtype=np.float64):
         rp, rq = rp[rows[0]:rows[1]], rq[rows[0]:rows[1]]
     out = np.empty((len(rp), len(cp)), dtype=dtype)
     chunk = max(1, 4_000_000 // max(1, len(cp)))
-    for r0 in range(0, len(rp), chunk):
+
+    def fill(r0):
         r1 = min(len(rp), r0 + chunk)
         out[r0:r1] = integral_values(rp[r0:r1, None], rq[r0:r1, None], cp[None, :], cq[None, :], seed, scale)
+
+    starts = range(0, len(rp), chunk)
+    if workers > 1 and len(starts) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(fill, starts))
+    else:
+        for r0 in starts:
+            fill(r0)
     return out
 
 
-def make_problem(O, V, seed, scale=0.02, blocks=BLOCKS):
+def make_problem(O, V, seed, scale=0.02, blocks=BLOCKS, workers=1):
     """Synthetic problem: Fock matrix (diag) and packed integral blocks."""
     eo, ev = orbital_energies(O, V, seed)
     f = np.diag(np.concatenate([eo, ev]))
     prob = dict(O=O, V=V, seed=seed, f=f)
     for b in blocks:
-        prob[b] = packed_block(b, O, V, seed, scale)
+        prob[b] = packed_block(b, O, V, seed, scale, workers=workers)
     return prob
 
 

@@ -76,8 +76,80 @@ def _write_input(path, n, nocc, ehf, eps, C, ao, mo):
             np.ascontiguousarray(x, dtype=np.float64).tofile(fh)
 
 
+def _random_ao(n, seed):
+    """8-fold symmetric (pq|rs) = sum_k L[pq,k] L[rs,k] with L[pq] = L[qp]."""
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal((n, n, 6)) * 0.3
+    L = 0.5 * (L + L.transpose(1, 0, 2))
+    Lm = L.reshape(n * n, 6)
+    return (Lm @ Lm.T).reshape(n, n, n, n)
+
+
+def test_reference_field_q_layout_after_rekey():
+    """The slices a reference Hartree_Fock_Solver hands to CCSD.
+
+    Restates, on random 8-fold symmetric AO integrals, the SCF's fill
+    field_Q(o1,o3)(o2,o4) = Calc_Integrals_On_The_Fly(o1,o3,o2,o4) =
+    (o1 o2|o3 o4) (hartree_fock_solver.cpp:798-802; electron 1 is the
+    E_index(orb1, orb2) pair, :74) and the re-key loops of :1045-1210 with
+    their four branch conditions, then checks the resulting Return_Field_Q(j,l)
+    is the (pq|jl) slice the drop-in's provider interface documents -- and that
+    the SCF's own layout, (jp|lq), is not."""
+    n = 5
+    ao = _random_ao(n, 3)
+    field_q = {(o1, o3): np.zeros((n, n)) for o1 in range(n) for o3 in range(n)}
+    for o1 in range(n):
+        for o2 in range(n):
+            for o3 in range(n):
+                for o4 in range(n):
+                    field_q[o1, o3][o2, o4] = ao[o1, o2, o3, o4]
+    temp_q = {(i, j): np.zeros((n, n)) for i in range(n) for j in range(i + 1)}
+    for o1 in range(n):
+        for o3 in range(n):
+            tt = field_q[o1, o3]                      # the broadcast slice [orb2][orb4]
+            for o2 in range(n):
+                for o4 in range(n):
+                    v = tt[o2, o4]
+                    if o2 <= o1:
+                        temp_q[o1, o2][o3, o4] = v
+                        temp_q[o1, o2][o4, o3] = v
+                    if o1 <= o2:
+                        temp_q[o2, o1][o3, o4] = v
+                        temp_q[o2, o1][o4, o3] = v
+                    if o4 <= o3:
+                        temp_q[o3, o4][o1, o2] = v
+                        temp_q[o3, o4][o2, o1] = v
+                    if o3 <= o4:
+                        temp_q[o4, o3][o1, o2] = v
+                        temp_q[o4, o3][o2, o1] = v
+    for j in range(n):
+        for l in range(j + 1):
+            assert np.array_equal(temp_q[j, l], ao[:, :, j, l]), (j, l)
+    pre = max(np.abs(field_q[j, l] - ao[:, :, j, l]).max() for j in range(n) for l in range(j + 1))
+    assert pre > 1e-2
+
+
+def test_frozen_core_count_matches_reference(tmp_path):
+    """Freeze_How_Meny_Orbitals (ccsd_memory_optimized.cpp:5203-5231): one
+    spatial orbital per nucleus with 2 < Z < 11, none otherwise."""
+    exe = _build_driver(tmp_path)
+    cases = {(1,): 0, (8,): 1, (11,): 0, (17,): 0, (2,): 0, (3,): 1, (10,): 1, (8, 1, 1): 1,
+             (8, 8, 1, 1): 2, (6, 17, 1): 1}
+    for zs, want in cases.items():
+        out = subprocess.run([exe, "--frozen"] + [str(z) for z in zs], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0, out.stderr
+        vals = dict(zip(out.stdout.split()[0::2], (int(v) for v in out.stdout.split()[1::2])))
+        assert vals["frozen_on"] == want, (zs, vals)
+        assert vals["frozen_off"] == 0
+
+
 @pytest.mark.gpu
 def test_dropin_class_matches_oracle(tmp_path):
+    """The class fed by a provider holding its AO integrals as the reference's
+    Hartree_Fock_Solver does (SCF fill, then the re-key get_Energy applies)
+    and by the direct MO constructor: both equal the oracle to 1e-10 Eh.  The
+    same provider without the re-key gives another energy (the layout matters
+    and the test sees it)."""
     n, nocc = 8, 3
     eps, mo = rhf_spin.random_rhf(n, nocc, 7)
     rng = np.random.default_rng(8)
@@ -94,8 +166,13 @@ def test_dropin_class_matches_oracle(tmp_path):
     prob = rhf_spin.problem_from_rhf(eps, mo, nocc)
     E, *_ = ccsd_ref.ccsd_energy(prob, toler=1e-12, max_iter=300, diis_dim=8)
     assert abs(float(vals["provider_corr"]) - E) < 1e-10
+    assert abs(float(vals["provider_total"]) - (E + ehf)) < 1e-10
     assert abs(float(vals["direct_corr"]) - E) < 1e-10
     assert abs(float(vals["direct_total"]) - (E + ehf)) < 1e-10
+    bad = subprocess.run([exe, inp, "prekey"], capture_output=True, text=True, timeout=120)
+    vb = dict(zip(bad.stdout.split()[0::2], bad.stdout.split()[1::2]))
+    wrong = vb.get("provider_corr", "nan")
+    assert not (np.isfinite(float(wrong)) and abs(float(wrong) - E) < 1e-6), (wrong, E)
 
 
 # -- config 1: H2O / 6-31G through the INCAR front-end and the drop-in class

@@ -13,6 +13,8 @@ what stays exact at any size:
     against the oracle at small sizes in test_gpu_parity.py);
   * the ring GEMM (Fill_W5, :6240): rows (i,a) of X = Tring W_mbej^T, from the
     engine's Tring and W_mbej buffers;
+  * the whole right-hand side at O=20/V=180 against the oracle's independent
+    packed C++ restatement (oracle/ccsd_cpu.cpp) on the host generator's blocks;
   * the converged FP64 solve at O=20/V=180 to 1e-10 (the plain FP64
     configuration): its amplitudes are a fixed point, D t = R(t), to the
     reported max|dt2|, and the returned energy is E(t*).
@@ -78,6 +80,32 @@ def test_full_size_products_and_first_energy(O, V, seed):
         ref = Tring[rows] @ WT.T
         err = np.abs(X[rows] - ref).max()
         assert err < 1e-12 * np.abs(ref).max(), err
+
+
+@pytest.mark.timeout(600)
+def test_residual_o20_v180_matches_cpu_oracle():
+    """The whole FP64 right-hand side at O=20 / V=180 (R1 and the packed R2)
+    against the oracle's independent packed C++ restatement
+    (oracle/ccsd_cpu.cpp cpu_ccsd_residual, itself equal to the dense numpy
+    oracle at small shapes), on the host generator's blocks, at random
+    amplitudes around the MP2 guess: every entry within 1e-12 of the largest."""
+    from oracle import cpu_build
+    capi = _capi()
+    O, V, seed = 20, 180, 20180
+    scale = 1.0 / np.sqrt(O * synth.p2(V))
+    print("case: residual O", O, "V", V, "seed", seed, flush=True)
+    with capi.Context.synthetic(O, V, seed, scale, diis_dim=8) as ctx:
+        t1m, t2m = ctx.get_amplitudes()
+        rng = np.random.default_rng(seed + 2)
+        t1 = 1e-3 * rng.standard_normal((O, V))
+        t2 = t2m * (1.0 + 0.1 * rng.standard_normal(t2m.shape))
+        g1, g2 = ctx.residual(t1, t2)
+    prob = synth.make_problem(O, V, seed, scale=scale, workers=16)
+    c1, c2 = cpu_build.residual(prob, t1, t2)
+    e2 = np.abs(g2 - c2).max() / np.abs(c2).max()
+    e1 = np.abs(g1 - c1).max() / np.abs(c1).max()
+    print("R2 max|d|/max %.2e  R1 max|d|/max %.2e" % (e2, e1), flush=True)
+    assert e2 < 1e-12 and e1 < 1e-12
 
 
 def test_fp64_solve_o20_v180_is_a_fixed_point():
