@@ -24,7 +24,7 @@ EXPORTED = (
     "ccsd_residual", "ccsd_energy", "ccsd_get_amplitudes", "ccsd_set_amplitudes", "ccsd_energy_trace",
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
     "ccsd_selftest_gemm", "ccsd_update_integrals",
-    "ccsd_host_alloc", "ccsd_host_free", "ccsd_create_rhf", "ccsd_ao_to_mo",
+    "ccsd_host_alloc", "ccsd_host_free", "ccsd_create_rhf", "ccsd_ao_to_mo", "ccsd_energy_noalloc",
     # include/ccsd_rhf.h (host RHF input producer)
     "ccsd_basis_size", "ccsd_rhf", "ccsd_ao_one_electron",
 )
@@ -100,6 +100,7 @@ def load():
     lib.ccsd_reset.argtypes = [vp]
     lib.ccsd_residual.argtypes = [vp, dp, dp, dp, dp]
     lib.ccsd_energy.argtypes = [vp, dp, dp, dp]
+    lib.ccsd_energy_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
     lib.ccsd_get_amplitudes.argtypes = [vp, dp, dp]
     lib.ccsd_set_amplitudes.argtypes = [vp, dp, dp]
     lib.ccsd_energy_trace.argtypes = [vp, dp, C.c_int]

@@ -173,6 +173,21 @@ int ccsd_residual(ccsd_ctx* c, const double* t1, const double* t2, double* r1, d
 /* Energy E = f_ia t_i^a + sum_{i<j,a<b} <ij||ab> tau_ij^ab at given amplitudes. */
 int ccsd_energy(ccsd_ctx* c, const double* t1, const double* t2, double* e);
 
+/* Allocation-free variant of ccsd_energy.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Every pointer is a device pointer owned by
+ * the caller: fov [O*V] (f_ia), oovv packed [P2(O)][P2(V)], t1 [O*V], t2
+ * packed [P2(O)][P2(V)], e [1] (receives E); stream is a hipStream_t (NULL =
+ * the null stream).  The kernels are enqueued on that stream and the call
+ * returns without synchronising; it calls no hipMalloc / hipFree and no
+ * allocating library routine.  Partial sums go to a fixed 64 KB device array
+ * per GPU, held until the process ends, whose address is resolved on the
+ * first call on each device: O above 128 (P2(O) + 1 > 8192 partials) returns
+ * CCSD_ERR_BUDGET.  Calls that overlap on one GPU share that array and race;
+ * order them (one stream, or events). */
+int ccsd_energy_noalloc(int n_occ, int n_virt, const double* fov, const double* oovv, const double* t1,
+                        const double* t2, double* e, void* stream);
+
 int ccsd_get_amplitudes(ccsd_ctx* c, double* t1, double* t2);
 int ccsd_set_amplitudes(ccsd_ctx* c, const double* t1, const double* t2);
 
