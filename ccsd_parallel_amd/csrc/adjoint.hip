@@ -11,9 +11,8 @@
 //    cotangents switched on -- the unrelaxed one- and two-particle density
 //    contractions -- plus dE/dtheta and the denominator term.
 //  The four-virtual block gradient (P2(V) x P2(V), 16 GB at O=40 V=300) is
-//  by default never held whole in HBM: it is recomputed in row slabs from tau
-//  and Lambda and streamed to the host (opt.recompute_v4 = 1; with 0 the whole
-//  block is formed in HBM in one pass).
+//  formed whole in HBM only where it fits (opt.recompute_v4): otherwise it is
+//  recomputed in row slabs from tau and Lambda and streamed to the host.
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
 //    until max|dLambda| < lambda_switch_factor * tol (or until it stalls:
@@ -691,9 +690,30 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
     if (delta < tol && fp64_after >= c->opt.final_fp64_sweeps) { conv = true; break; }
   }
 
-  // gradient pass
+  // gradient pass.  The four-virtual block dE/d<ef||ab> = 1/2 (tau^T lam2 +
+  // lam2^T tau) needs only tau and the converged Lambda, so its slabs are formed
+  // on a stream of their own and copied to the host on a second one while the
+  // main stream runs the reverse pass with the integral cotangents; the other
+  // blocks go out on a third stream once that pass and its folds are done.
+  // (All streams at the default priority; page-locked destinations make the
+  // copies DMA transfers.)
   ThetaBars th;
   std::vector<void*> tmp;
+  hipStream_t vs = nullptr, cs = nullptr, cs2 = nullptr;
+  hipEvent_t ev_lam = nullptr, ev_fold = nullptr, ev_comp[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+  double* slab[2] = {nullptr, nullptr};
+  auto teardown = [&]() {
+    for (hipStream_t q : {vs, cs, cs2})
+      if (q) (void)hipStreamSynchronize(q);
+    (void)hipStreamSynchronize(s);
+    for (double* p : slab)
+      if (p) (void)hipFree(p);
+    for (hipEvent_t e : {ev_lam, ev_fold, ev_comp[0], ev_comp[1], ev_copy[0], ev_copy[1]})
+      if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : {vs, cs, cs2})
+      if (q) (void)hipStreamDestroy(q);
+    for (void* p : tmp) (void)hipFree(p);
+  };
   auto talloc = [&](double** p, long n) -> int {
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, (size_t)std::max<long>(n, 1) * 8);
@@ -707,17 +727,82 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
     return CCSD_OK;
   };
   int rc = CCSD_OK;
-  auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
   if ((rc = talloc(&th.oooo, po * po)) || (rc = talloc(&th.ooov, po * ov)) || (rc = talloc(&th.oovv, po * pv)) ||
       (rc = talloc(&th.ovvv, ov * pv)) || (rc = talloc(&th.G, ov * ov)) || (rc = talloc(&th.Doovv, ov * ov)) ||
       (rc = talloc(&th.oovv_ou, O * O * pv)) || (rc = talloc(&th.oovv_vu, po * VV)) ||
       (rc = talloc(&th.ovvv_u, ov * VV)) || (rc = talloc(&th.fvv, VV)) || (rc = talloc(&th.fov, ov)) ||
       (rc = talloc(&th.f, N + 8))) {
-    cleanup();
+    teardown();
     return rc;
   }
+  // Four-virtual block policy (recompute_v4): 0 forms the whole P2(V)^2 block in
+  // HBM, 1 streams it through two 512 MB row-slab buffers, < 0 (default) forms it
+  // whole when free memory holds it with 4 GB to spare and streams it otherwise.
+  // Whole block: its row slabs are formed on the main stream before the reverse
+  // pass (at full speed) and each is copied out on cs while that pass runs.
+  // Slabs: they are formed on vs, each waiting for the copy of the slab two
+  // before it, so the reverse pass on the main stream is never held back.
+  const long slab_rows = std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv));
+  const size_t whole_bytes = (size_t)pv * pv * 8;
+  bool whole = false;
+  if (c->opt.recompute_v4 != 1) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      teardown();
+      return CCSD_ERR_HIP;
+    }
+    whole = c->opt.recompute_v4 == 0 ? whole_bytes <= fr : whole_bytes + (4ull << 30) <= fr;
+    if (!whole && c->opt.recompute_v4 == 0) {
+      c->last_error = "four-virtual gradient block does not fit; set recompute_v4";
+      teardown();
+      return CCSD_ERR_BUDGET;
+    }
+  }
+  const int nbuf = whole ? 1 : (slab_rows >= pv ? 1 : 2);
+  for (int b = 0; b < nbuf; ++b)
+    if (hipMalloc(&slab[b], whole ? whole_bytes : (size_t)slab_rows * pv * 8) != hipSuccess) {
+      teardown();
+      return whole ? CCSD_ERR_BUDGET : CCSD_ERR_HIP;
+    }
+  bool ok = hipStreamCreateWithFlags(&vs, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&cs2, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&ev_lam, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&ev_fold, hipEventDisableTiming) == hipSuccess;
+  for (int b = 0; b < 2 && ok; ++b)
+    ok = hipEventCreateWithFlags(&ev_comp[b], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ev_copy[b], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventRecord(ev_lam, s) == hipSuccess && hipStreamWaitEvent(vs, ev_lam, 0) == hipSuccess;
+  if (!ok) {
+    teardown();
+    return CCSD_ERR_HIP;
+  }
+  {
+    const double* lam2 = A->lam + ov;
+    const GemmWorkspace none{};  // vs runs beside the main stream's GEMMs, which hold A->ws
+    hipStream_t gs = whole ? s : vs;
+    const GemmWorkspace& wsl = whole ? A->ws : none;
+    long k = 0;
+    for (long r0 = 0; r0 < pv && ok; r0 += slab_rows, ++k) {
+      const long nr = std::min(slab_rows, pv - r0);
+      const int b = (int)(k % nbuf);
+      double* dst = whole ? slab[0] + r0 * pv : slab[b];
+      if (!whole && k >= nbuf && hipStreamWaitEvent(vs, ev_copy[b], 0) != hipSuccess) ok = false;
+      ok = ok && gemm<double>(G64(c->tau + r0, 1, pv, lam2, pv, 1, dst, pv, 1, nr, pv, po, 0.5, 0.0), wsl, gs) ==
+                     hipSuccess;
+      ok = ok && gemm<double>(G64(lam2 + r0, 1, pv, c->tau, pv, 1, dst, pv, 1, nr, pv, po, 0.5, 1.0), wsl, gs) ==
+                     hipSuccess;
+      ok = ok && hipEventRecord(ev_comp[b], gs) == hipSuccess && hipStreamWaitEvent(cs, ev_comp[b], 0) == hipSuccess &&
+           hipMemcpyAsync(g.vvvv + r0 * pv, dst, (size_t)nr * pv * 8, hipMemcpyDeviceToHost, cs) == hipSuccess &&
+           hipEventRecord(ev_copy[b], cs) == hipSuccess;
+    }
+    if (!ok) {
+      teardown();
+      return CCSD_ERR_HIP;
+    }
+  }
   rc = reverse_pass(c, A->lam, &th);
-  if (rc != CCSD_OK) { cleanup(); return rc; }
+  if (rc != CCSD_OK) { teardown(); return rc; }
   // dE/dtheta: <ij||ab> pairs with tau, f_ov with t1
   LAUNCH(scale_add_kernel, po * pv, th.oovv, c->tau, 1.0, po * pv);
   LAUNCH(scale_add_kernel, ov, th.fov, t1, 1.0, ov);
@@ -731,89 +816,19 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(po * (po - 1) / 2)), dim3(kThreads), 0, s, th.oooo, po);
   // denominator term on the Fock diagonal
   hipLaunchKernelGGL(fdiag_kernel, dim3(N), dim3(kThreads), 0, s, th.f, A->lam, c->amp, d);
-  if (hipGetLastError() != hipSuccess) { cleanup(); return CCSD_ERR_HIP; }
-
-  // Copies to the host go on a second stream (default priority) so they overlap
-  // the four-virtual slab GEMMs; page-locked destination buffers make them DMA
-  // transfers.  The derived-layout cotangents stay allocated until the end.
-  hipStream_t cs = nullptr;
-  hipEvent_t ev_fold = nullptr, ev_comp[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
-  double* slab[2] = {nullptr, nullptr};
-  auto teardown = [&]() {
-    if (cs) (void)hipStreamSynchronize(cs);
-    (void)hipStreamSynchronize(s);
-    for (double* p : slab)
-      if (p) (void)hipFree(p);
-    for (hipEvent_t e : {ev_fold, ev_comp[0], ev_comp[1], ev_copy[0], ev_copy[1]})
-      if (e) (void)hipEventDestroy(e);
-    if (cs) (void)hipStreamDestroy(cs);
-    cleanup();
-  };
-  bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&ev_fold, hipEventDisableTiming) == hipSuccess;
-  for (int b = 0; b < 2 && ok; ++b)
-    ok = hipEventCreateWithFlags(&ev_comp[b], hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&ev_copy[b], hipEventDisableTiming) == hipSuccess;
+  if (hipGetLastError() != hipSuccess) { teardown(); return CCSD_ERR_HIP; }
   std::vector<double> fov(ov), fvv(VV), fmi(O * O), fd(N);
   auto d2h = [&](double* h, const double* dv, long n) {
-    return hipMemcpyAsync(h, dv, (size_t)n * 8, hipMemcpyDeviceToHost, cs) == hipSuccess;
+    return hipMemcpyAsync(h, dv, (size_t)n * 8, hipMemcpyDeviceToHost, cs2) == hipSuccess;
   };
-  ok = ok && hipEventRecord(ev_fold, s) == hipSuccess && hipStreamWaitEvent(cs, ev_fold, 0) == hipSuccess;
+  ok = hipEventRecord(ev_fold, s) == hipSuccess && hipStreamWaitEvent(cs2, ev_fold, 0) == hipSuccess;
   ok = ok && d2h(g.oooo, th.oooo, po * po) && d2h(g.ooov, th.ooov, po * ov) && d2h(g.oovv, th.oovv, po * pv) &&
        d2h(g.ovov, ovovb, ov * ov) && d2h(g.ovvv, th.ovvv, ov * pv) && d2h(fov.data(), th.fov, ov) &&
        d2h(fvv.data(), th.fvv, VV) && d2h(fmi.data(), A->Fmib, O * O) && d2h(fd.data(), th.f, N);
-  if (!ok) {
-    teardown();
-    return CCSD_ERR_HIP;
-  }
-
-  // four-virtual block: dE/d<ef||ab> = 1/2 (tau^T lam2 + lam2^T tau).  With
-  // recompute_v4 (default) it is recomputed in 512 MB row slabs, computed into
-  // two alternating device buffers and streamed to the host; without, the
-  // whole P2(V)^2 block is formed in HBM in one pass (CCSD_ERR_BUDGET when it
-  // does not fit in the free device memory)
-  {
-    const double* lam2 = A->lam + ov;
-    const long slab_rows =
-        c->opt.recompute_v4 ? std::max<long>(1, std::min<long>(pv, (512L << 20) / 8 / pv)) : pv;
-    const int nbuf = slab_rows >= pv ? 1 : 2;
-    if (!c->opt.recompute_v4) {
-      size_t fr = 0, tot = 0;
-      if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-        teardown();
-        return CCSD_ERR_HIP;
-      }
-      if ((size_t)pv * pv * 8 > fr) {
-        c->last_error = "four-virtual gradient block does not fit; set recompute_v4";
-        teardown();
-        return CCSD_ERR_BUDGET;
-      }
-    }
-    for (int b = 0; b < nbuf; ++b)
-      if (hipMalloc(&slab[b], (size_t)slab_rows * pv * 8) != hipSuccess) {
-        teardown();
-        return c->opt.recompute_v4 ? CCSD_ERR_HIP : CCSD_ERR_BUDGET;
-      }
-    long k = 0;
-    for (long r0 = 0; r0 < pv && rc == CCSD_OK; r0 += slab_rows, ++k) {
-      const long nr = std::min(slab_rows, pv - r0);
-      const int b = (int)(k & 1);
-      if (k >= 2 && hipStreamWaitEvent(s, ev_copy[b], 0) != hipSuccess) rc = CCSD_ERR_HIP;
-      if (rc == CCSD_OK)
-        rc = run_gemm(c, G64(c->tau + r0, 1, pv, lam2, pv, 1, slab[b], pv, 1, nr, pv, po, 0.5, 0.0), &A->ws);
-      if (rc == CCSD_OK)
-        rc = run_gemm(c, G64(lam2 + r0, 1, pv, c->tau, pv, 1, slab[b], pv, 1, nr, pv, po, 0.5, 1.0), &A->ws);
-      if (rc == CCSD_OK &&
-          (hipEventRecord(ev_comp[b], s) != hipSuccess || hipStreamWaitEvent(cs, ev_comp[b], 0) != hipSuccess ||
-           hipMemcpyAsync(g.vvvv + r0 * pv, slab[b], (size_t)nr * pv * 8, hipMemcpyDeviceToHost, cs) != hipSuccess ||
-           hipEventRecord(ev_copy[b], cs) != hipSuccess))
-        rc = CCSD_ERR_HIP;
-    }
-    if (rc == CCSD_OK && (hipStreamSynchronize(cs) != hipSuccess || hipStreamSynchronize(s) != hipSuccess))
-      rc = CCSD_ERR_HIP;
-    teardown();
-    if (rc != CCSD_OK) return rc;
-  }
+  for (hipStream_t q : {cs2, cs, vs, s})
+    ok = ok && hipStreamSynchronize(q) == hipSuccess;
+  teardown();
+  if (!ok) return CCSD_ERR_HIP;
 
   // Fock cotangent: ov block (used as f[i][O+a]), oo / vv off-diagonal, diagonal from D; symmetrised
   std::vector<double> G((size_t)N * N, 0.0);

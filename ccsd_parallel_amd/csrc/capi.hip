@@ -240,7 +240,7 @@ int ccsd_default_options(ccsd_options* o) {
   o->final_fp64_sweeps = 3;
   o->diis_dim = 8;
   o->diis_start = 1;
-  o->recompute_v4 = 1;
+  o->recompute_v4 = -1;
   o->verbose = 0;
   o->device = 0;
   o->hbm_budget_bytes = 0;

@@ -64,11 +64,14 @@ typedef struct ccsd_options {
   int final_fp64_sweeps;    /* minimum FP64 iterations after the switch (default 3) */
   int diis_dim;             /* 0 or 1: plain Jacobi as the reference; >= 2: DIIS subspace (default 8) */
   int diis_start;           /* first iteration allowed to extrapolate (default 1) */
-  int recompute_v4;         /* backward: 1 (default) recomputes the four-virtual gradient block
-                               dE/d<ab||cd> = 1/2 (tau^T Lambda2 + Lambda2^T tau) in 512 MB row slabs,
-                               each streamed to the host while the next is formed; 0 forms the whole
-                               P2(V)^2 block in HBM in one pass (16 GB at O=40/V=300;
-                               CCSD_ERR_BUDGET when it does not fit in free device memory) */
+  int recompute_v4;         /* backward, four-virtual gradient block dE/d<ab||cd> =
+                               1/2 (tau^T Lambda2 + Lambda2^T tau), P2(V)^2 doubles (16 GB at
+                               O=40/V=300): 0 forms it whole in HBM (CCSD_ERR_BUDGET when it does not
+                               fit in free device memory); 1 recomputes it in 512 MB row slabs through
+                               two buffers, each slab streamed to the host while the next is formed;
+                               < 0 (default) forms it whole when free memory holds it with 4 GB to
+                               spare, else as 1.  Either way its copy to the host overlaps the rest of
+                               the gradient pass. */
   int verbose;              /* print "Energi: E Steg: n Time/iter: s" lines like the reference */
   int device;               /* HIP device ordinal */
   size_t hbm_budget_bytes;  /* 0 = 90 % of the device's free memory */

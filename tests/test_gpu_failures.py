@@ -141,13 +141,14 @@ def test_overflowing_amplitudes_are_reported():
 
 
 def test_backward_four_virtual_block_modes_agree():
-    """recompute_v4 = 1 (row slabs streamed to the host) and 0 (the whole block
-    in HBM) give the same gradient; the slab size is 512 MB, so O=20/V=180
-    (P2(V) = 16110, 2.1 GB) runs five slabs."""
+    """recompute_v4 = 1 (row slabs through two buffers, formed on a side
+    stream), 0 (the whole block in HBM) and the default (whole here: 2.1 GB
+    fits) give the same gradient; the slab size is 512 MB, so O=20/V=180
+    (P2(V) = 16110) runs five slabs."""
     capi = _capi()
     O, V = 20, 180
     g = {}
-    for rv4 in (1, 0):
+    for rv4 in (1, 0, -1):
         with capi.Context.synthetic(O, V, 20180, _scale(O, V), diis_dim=8, recompute_v4=rv4) as ctx:
             ctx.solve(1e-10, 200)
             g[rv4], info = ctx.backward()
@@ -155,3 +156,4 @@ def test_backward_four_virtual_block_modes_agree():
     for k in g[1]:
         scale = max(1e-300, np.abs(g[1][k]).max())
         assert np.abs(g[0][k] - g[1][k]).max() <= 1e-13 * scale, k
+        assert np.abs(g[-1][k] - g[1][k]).max() <= 1e-13 * scale, k
