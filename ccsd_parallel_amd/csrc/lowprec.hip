@@ -192,26 +192,44 @@ __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, f
   }
 }
 
+// A wave-uniform pointer held in scalar registers: loads then take the
+// scalar-base form (64-bit base + 32-bit per-lane offset), and the compiler
+// cannot fold the per-lane offset into hoisted 64-bit address registers.
+__device__ __forceinline__ const char* uniform_ptr(const char* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const char*)(((uint64_t)hi << 32) | lo);
+}
+
 // C[m][n] = sum_k A[m][k] B[n][k] with A = Ahi + Alo, B = Bhi + Blo, dropping
 // lo*lo: three v_mfma_f32_16x16x32_bf16 per 16x16x32 step, FP32 accumulation.
 //
 // Block tile BM x BN over WM x WN waves (wave tile (BM/WM) x (BN/WN)), k-tiles
-// of 32.  Each thread moves 16-byte pieces of the interleaved rows (8 per row
-// per k-tile: 4 hi + 4 lo) through registers into double-buffered LDS.  A row
-// keeps its global order (128 bytes: hi0 lo0 hi1 lo1 .. lo3) with the 16-byte
-// chunk index XOR-swizzled by (row & 7) ^ ((row >> 3) & 1), and no padding: 16
-// lanes store two whole rows (256 contiguous bytes), and the 16 rows of one
-// MFMA operand fetch (rows 16i .. 16i + 15, one chunk each) read without bank
-// conflicts: a lane group's 8 rows 0..7 and 8..15 take distinct chunks, and
-// rows of equal parity in the two halves take chunks of opposite parity.  (The
-// swizzle (row >> 1) & 7 measured two-way conflicted reads on gfx950, 8 LDS
-// cycles per 16-byte wave read instead of 4.)  The next k-tile's loads are issued
-// unconditionally before the current tile's MFMAs (the last iteration reloads
-// its own tile into the idle buffer) so they stay in flight across the
-// compute; rows past M / N are clamped to the last row (they only feed
-// accumulator rows that are never stored).  At 256 x 256 the LDS is 128 KB and
-// a block holds one CU; the 128 x 128 tile (64 KB, 4 waves) is compiled for two
-// resident blocks per CU.
+// of 32, LDS double-buffered in two arrays per operand (256 x 256: 128 KB, one
+// block per CU; 128 x 128 on 4 waves: 64 KB, two blocks per CU).
+//
+// Operands go from global memory straight into LDS (global_load_lds_dwordx4):
+// lane l of a wave writes the 16 bytes at the wave's LDS base + 16 l, so
+// piece u = it*NT + tid lands at byte 16 u of the stage, i.e. tile row u/8,
+// slot u%8, and the lane loads the global chunk that belongs in that slot.  A
+// row keeps its 128 bytes (hi0 lo0 hi1 lo1 .. lo3 of one 32-deep k-tile) with
+// chunk q at slot q ^ (row & 7) ^ ((row >> 3) & 1): 16 lanes store two whole
+// rows and the 16 rows of one MFMA operand fetch read without bank conflicts
+// (within each half of the 16 rows the slots are distinct, and rows of equal
+// parity in the two halves take slots of opposite parity).  Since every pass
+// moves NT/8 rows (a multiple of 16), a thread's chunk index is the same in
+// every pass.  Each load is a scalar base (tile row start + k offset) plus a
+// 32-bit per-lane offset that holds the row, clamped to the last row (rows
+// past M / N only feed accumulator rows that are never stored); no registers
+// stage the data and no LDS store instructions run.
+//
+// Per k-tile: wait for this wave's loads of the tile, barrier (every wave's
+// loads have landed and every wave is done with the previous tile, whose
+// buffer is refilled next), issue the next tile's loads, then the MFMAs.  The
+// three products of a 16 x 16 output block are issued as three passes over the
+// wave's row blocks, so dependent MFMAs are TM instructions apart.  Measured
+// at 12000^3 on MI355X against staging the loads through registers (the
+// previous form of this kernel): 7.97 -> 7.26 ms.
 //
 // Tile order: XCD-aware (linear workgroup x runs on XCD x % 8, so XCD j takes a
 // contiguous range of tiles); with group > 0 the tiles of that range are
@@ -223,14 +241,14 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   constexpr int BK = 32;
   constexpr int RP = 2 * BK;  // row pitch in bf16 elements: 128 bytes, hi and lo interleaved per 8
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
-  // 16-byte pieces per thread per k-tile (the last pass partial when BM * 8 or
-  // BN * 8 is not a multiple of the thread count)
+  // 16-byte pieces per thread per k-tile; a partial last pass drops whole waves
   constexpr int CA = (BM * 8 + NT - 1) / NT, CB = (BN * 8 + NT - 1) / NT;
-  constexpr bool FA = CA * NT == BM * 8, FB = CB * NT == BN * 8;
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * RP];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * RP];
-  // LDS slot of 16-byte chunk q of tile row m
-  auto slot = [](int m, int q) { return m * RP + ((q ^ (m & 7) ^ ((m >> 3) & 1)) << 3); };
+  static_assert((NT / 8) % 16 == 0, "a pass must cover whole 16-row swizzle periods");
+  static_assert((BM * 8) % 64 == 0 && (BN * 8) % 64 == 0, "passes end on wave boundaries");
+  __shared__ __attribute__((aligned(1024))) uint16_t sA0[BM * RP];
+  __shared__ __attribute__((aligned(1024))) uint16_t sA1[BM * RP];
+  __shared__ __attribute__((aligned(1024))) uint16_t sB0[BN * RP];
+  __shared__ __attribute__((aligned(1024))) uint16_t sB1[BN * RP];
 
   int tile, kt0 = 0, kt1 = p.Kp / 32;  // k-tile range of this unit
   float* tail_out = nullptr;           // this unit's FP32 partial (split tail tiles)
@@ -256,25 +274,21 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WN) * (BM / WM), wn0 = (wave % WN) * (BN / WN);
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
 
-  // piece u = it*NT + tid: row u/8, chunk q = u%8 (k-slice q/2, hi for q even, lo for q odd)
-  const uint16_t* pa[CA];
-  const uint16_t* pb[CB];
-  int sa[CA], sb[CB];
+  // per-lane byte offsets from the tile's first row: row (it*NT/8 + tid/8), clamped, chunk q
+  const int mr = tid >> 3, sl = tid & 7;
+  const int q = sl ^ (mr & 7) ^ ((mr >> 3) & 1);
+  uint32_t offa[CA], offb[CB];
 #pragma unroll
-  for (int it = 0; it < CA; ++it) {
-    const int u = min(it * NT + tid, BM * 8 - 1), m = u >> 3, q = u & 7;
-    pa[it] = p.A + (long)min(m0 + m, p.M - 1) * 2 * p.lda + q * 8;
-    sa[it] = slot(m, q);
-  }
+  for (int it = 0; it < CA; ++it)
+    offa[it] = (uint32_t)(((long)(min(m0 + it * (NT / 8) + mr, p.M - 1) - m0) * 2 * p.lda + q * 8) * 2);
 #pragma unroll
-  for (int it = 0; it < CB; ++it) {
-    const int u = min(it * NT + tid, BN * 8 - 1), n = u >> 3, q = u & 7;
-    pb[it] = p.B + (long)min(n0 + n, p.N - 1) * 2 * p.ldb + q * 8;
-    sb[it] = slot(n, q);
-  }
-  // threads past the last piece of a partial pass
-  const bool la = FA || (CA - 1) * NT + tid < BM * 8, lb = FB || (CB - 1) * NT + tid < BN * 8;
+  for (int it = 0; it < CB; ++it)
+    offb[it] = (uint32_t)(((long)(min(n0 + it * (NT / 8) + mr, p.N - 1) - n0) * 2 * p.ldb + q * 8) * 2);
+  const uint16_t* rowa = p.A + (long)m0 * 2 * p.lda;
+  const uint16_t* rowb = p.B + (long)n0 * 2 * p.ldb;
+
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -282,94 +296,67 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
   const int nk = kt1 - kt0;
-  // global -> registers for k-tile t (clamped: the tail reloads the last tile into the idle buffer)
-  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-  u32x4 ra[CA], rb[CB];
-  auto load = [&](int t) {
-    const long kn = 2L * (kt0 + min(t, nk - 1)) * BK;
+  auto issue = [&](int t, uint16_t* la, uint16_t* lb) {
+    const char* ba = uniform_ptr((const char*)(rowa + 2L * (kt0 + t) * BK));
+    const char* bb = uniform_ptr((const char*)(rowb + 2L * (kt0 + t) * BK));
 #pragma unroll
     for (int it = 0; it < CA; ++it)
-      if (it < CA - 1 || la) ra[it] = *(const u32x4*)(pa[it] + kn);
+      if ((it + 1) * NT <= BM * 8 || it * NT + wv * 64 < BM * 8)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(ba + offa[it]),
+                                         (__attribute__((address_space(3))) void*)(la + (it * NT + wv * 64) * 8), 16,
+                                         0, 0);
 #pragma unroll
     for (int it = 0; it < CB; ++it)
-      if (it < CB - 1 || lb) rb[it] = *(const u32x4*)(pb[it] + kn);
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int it = 0; it < CA; ++it)
-      if (it < CA - 1 || la) *(u32x4*)&sA[buf][sa[it]] = ra[it];
-#pragma unroll
-    for (int it = 0; it < CB; ++it)
-      if (it < CB - 1 || lb) *(u32x4*)&sB[buf][sb[it]] = rb[it];
+      if ((it + 1) * NT <= BN * 8 || it * NT + wv * 64 < BN * 8)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(bb + offb[it]),
+                                         (__attribute__((address_space(3))) void*)(lb + (it * NT + wv * 64) * 8), 16,
+                                         0, 0);
   };
   // operand fetch: lane l reads tile row 16i + (l & 15) (a multiple of 16 plus
   // l & 15, so its swizzle is (l & 7) ^ ((l >> 3) & 1)), k-slice l >> 4
   const int swz = (lane & 7) ^ ((lane >> 3) & 1), ks = lane >> 4;
   const int oh = ((2 * ks) ^ swz) << 3, ol = ((2 * ks + 1) ^ swz) << 3;
-  struct AFrag { bf16x8_t h[TM], l[TM]; };
-  auto read_a = [&](int buf, AFrag& f) {
+  auto compute = [&](const uint16_t* a, const uint16_t* b) {
+    bf16x8_t ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int r = (wm0 + i * 16 + (lane & 15)) * RP;
-      f.h[i] = *(const bf16x8_t*)&sA[buf][r + oh];
-      f.l[i] = *(const bf16x8_t*)&sA[buf][r + ol];
+      ah[i] = *(const bf16x8_t*)&a[r + oh];
+      al[i] = *(const bf16x8_t*)&a[r + ol];
     }
-  };
-  auto read_b = [&](int buf, int j, bf16x8_t& bh, bf16x8_t& bl) {
-    const int r = (wn0 + j * 16 + (lane & 15)) * RP;
-    bh = *(const bf16x8_t*)&sB[buf][r + oh];
-    bl = *(const bf16x8_t*)&sB[buf][r + ol];
-  };
-  auto mma = [&](const AFrag& f, int j, bf16x8_t bh, bf16x8_t bl) {
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.l[i], bh, acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.h[i], bl, acc[i][j], 0, 0, 0);
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.h[i], bh, acc[i][j], 0, 0, 0);
-    }
-  };
-
-  // Each iteration first stores tile kt+1 (loaded during the previous
-  // iteration) into the idle buffer, which the previous barrier released,
-  // then issues the loads of tile kt+2 and computes tile kt: the LDS stores
-  // overlap the MFMAs instead of sitting between the last MFMA and the
-  // barrier.  The tile's one barrier sits before its last column block, whose
-  // B operand is read before it (the buffer is refilled behind the barrier):
-  // behind the barrier each wave reads tile kt+1's A operands into the other
-  // register set while the last MFMAs of tile kt run.  (A second global-load
-  // register stage does not fit the 256 registers of 2 waves per SIMD.)
-  auto step = [&](int kt, const AFrag& cur, AFrag& nxt) {
-    const int buf = kt & 1;
-    store(buf ^ 1);
-    // keep the loads below the stores: hoisted above them, the new tile would
-    // need a second register set while the old one still waits to be stored
-    __builtin_amdgcn_sched_barrier(0);
-    load(kt + 2);
-    bf16x8_t bh, bl;
+    for (int j = 0; j < TN; ++j) {
+      const int r = (wn0 + j * 16 + (lane & 15)) * RP;
+      const bf16x8_t bh = *(const bf16x8_t*)&b[r + oh], bl = *(const bf16x8_t*)&b[r + ol];
 #pragma unroll
-    for (int j = 0; j < TN - 1; ++j) {
-      read_b(buf, j, bh, bl);
-      mma(cur, j, bh, bl);
+      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
     }
-    read_b(buf, TN - 1, bh, bl);
-    __syncthreads();
-    // after the last tile this reads the idle buffer's reloaded copy of it:
-    // in bounds, and never used
-    read_a(buf ^ 1, nxt);
-    mma(cur, TN - 1, bh, bl);
   };
-  AFrag f0, f1;
-  load(0);
-  store(0);
-  load(1);
-  __syncthreads();
-  read_a(0, f0);
+  // vmcnt(0): every load this wave issued (the current tile's) has written LDS
+  constexpr int kVmcnt0 = 0x3f70;
+  issue(0, sA0, sB0);
   int kt = 0;
   for (; kt + 2 <= nk; kt += 2) {
-    step(kt, f0, f1);
-    step(kt + 1, f1, f0);
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    __syncthreads();
+    issue(kt + 1, sA1, sB1);
+    __builtin_amdgcn_sched_barrier(0);  // the next tile's loads go out before this tile's MFMAs
+    compute(sA0, sB0);
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    __syncthreads();
+    if (kt + 2 < nk) issue(kt + 2, sA0, sB0);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(sA1, sB1);
   }
-  if (kt < nk) step(kt, f0, f1);
+  if (kt < nk) {
+    __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    __syncthreads();
+    compute(sA0, sB0);
+  }
   if (tail_out) {
     // whole BM x BN tile in tile coordinates; the reduction drops rows and
     // columns past M and N
