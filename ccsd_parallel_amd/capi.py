@@ -37,7 +37,7 @@ class Options(C.Structure):
         ("precision", C.c_int), ("switch_threshold", C.c_double), ("final_fp64_sweeps", C.c_int),
         ("diis_dim", C.c_int), ("diis_start", C.c_int), ("recompute_v4", C.c_int), ("verbose", C.c_int),
         ("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("diis_reset_on_switch", C.c_int),
-        ("switch_factor", C.c_double), ("lambda_switch_factor", C.c_double),
+        ("switch_factor", C.c_double), ("lambda_switch_factor", C.c_double), ("lambda_tol", C.c_double),
     ]
 
 

@@ -15,7 +15,7 @@
 //  recomputed in row slabs from tau and Lambda and streamed to the host.
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
-//    until max|dLambda| < lambda_switch_factor * tol (or until it stalls:
+//    until max|dLambda| < lambda_switch_factor * lambda_tol (or until it stalls:
 //    no new smallest max|dLambda| in kLowpStall iterations), then at least
 //    final_fp64_sweeps FP64 iterations -- the forward's schedule, with the
 //    DIIS subspace kept across the switch.  The gradient pass is always FP64.
@@ -101,14 +101,15 @@ __global__ void zero_kernel(double* x, long n) { GRID_LOOP(i, n) x[i] = 0.0; }
 __global__ void scale_add_kernel(double* y, const double* x, double a, long n) { GRID_LOOP(i, n) y[i] += a * x[i]; }
 
 // out[(i,j)][c] = sgn(i,j) in[pidx][c]   (leading pair axis over n items unpacked)
-__global__ void unpack_pair_rows_kernel(double* out, const double* in, int n, long cols) {
+template <typename T>
+__global__ void unpack_pair_rows_kernel(T* out, const double* in, int n, long cols) {
   const long tot = (long)n * n * cols;
   GRID_LOOP(idx, tot) {
     long ij = idx / cols, cc = idx % cols;
     int i = (int)(ij / n), j = (int)(ij % n);
     long k;
     double sg = sgn_pair(i, j, k);
-    out[idx] = (sg == 0.0) ? 0.0 : sg * in[k * cols + cc];
+    out[idx] = (T)((sg == 0.0) ? 0.0 : sg * in[k * cols + cc]);
   }
 }
 // out[r][(i,j)] = sgn(i,j) in[r][pidx]
@@ -169,7 +170,8 @@ __global__ void outer_add_kernel(double* C, const double* x, const double* y, lo
 }
 
 // S_bar[(m,b,e)][j] = WT_bar[(j,b),(m,e)];  S_bar[(m,a,e)][m] -= Fae_bar[a][e]
-__global__ void sbar_kernel(double* Sb, const double* WTb, const double* Faeb, Dims d) {
+template <typename T>
+__global__ void sbar_kernel(T* Sb, const double* WTb, const double* Faeb, Dims d) {
   const long n = (long)d.O * d.V * d.V * d.O;
   GRID_LOOP(idx, n) {
     const int j = (int)(idx % d.O);
@@ -180,7 +182,7 @@ __global__ void sbar_kernel(double* Sb, const double* WTb, const double* Faeb, D
     const int m = (int)(t / d.V);
     double v = WTb[((long)j * d.V + b) * d.ov + (long)m * d.V + e];
     if (j == m) v -= Faeb[(long)b * d.V + e];
-    Sb[idx] = v;
+    Sb[idx] = (T)v;
   }
 }
 
@@ -375,7 +377,10 @@ struct ThetaBars {
 // Reverse pass of compute_rhs at the forward intermediates held in c (computed
 // at the converged amplitudes) with cotangents (lam1, lam2).  Writes
 // (dG/dT)^T lam into A->tb; with th != nullptr also accumulates the integral
-// and Fock cotangents.
+// and Fock cotangents.  With lowp (Lambda iterations before the FP64 sweeps)
+// the five O(N^6) products run as BF16x3 GEMMs and the secondary products
+// with O(N^5) work run on FP32 copies of their operands (lowprec_adj_setup,
+// lowprec_adj_lambda), each accumulating into the FP64 cotangents.
 static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = false) {
   AdjState* A = c->adj;
   const Dims& d = c->d;
@@ -387,7 +392,22 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   double* t1b = A->tb;
   double* t2b = A->tb + ov;
   const GemmWorkspace* W = &A->ws;
+  LowPrec* L = lowp ? c->lp : nullptr;
   auto zero = [&](double* x, long n) { LAUNCH(zero_kernel, n, x, n); };
+  // FP32 product into g's FP64 destination (the adjoint's workspace: batch reductions need it)
+  auto f32 = [&](const GemmDesc<float>& f) -> int {
+    hipError_t e = gemm<float>(f, *W, s);
+    if (e != hipSuccess) {
+      c->last_error = hipGetErrorString(e);
+      return CCSD_ERR_HIP;
+    }
+    return CCSD_OK;
+  };
+  // g in FP64, or (lowp) on the FP32 operands A32 / B32 laid out as g's
+  auto mm = [&](const GemmDesc<double>& g, const float* A32, const float* B32) -> int {
+    return lowp ? f32(f32_desc(g, A32, B32)) : run_gemm(c, g, W);
+  };
+  if (lowp) TRY(lowprec_adj_lambda(c, lam));
   zero(A->tb, c->nvec);
   zero(A->taub, po * pv);
   zero(A->T2ub, O * O * pv);
@@ -404,16 +424,24 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   zero(A->WTb, ov * ov);
 
   // -- assembly: R2 = Rp + P(ij)P(ab) X + P(ab) Y + P(ij) Z
-  if (!lowp) k_unpack_ring(A->Xb, lam2, d, s);  // low precision: unpacked into the BF16 operand below
-  k_unpack_vir(A->Yb, lam2, d, s);
-  k_unpack_occ(A->Zb, A->Zb, lam2, lam2, d, s);
+  if (!lowp) {  // low precision: unpacked into the BF16 ring operand and the FP32 Yb / Zb
+    k_unpack_ring(A->Xb, lam2, d, s);
+    k_unpack_vir(A->Yb, lam2, d, s);
+    k_unpack_occ(A->Zb, A->Zb, lam2, lam2, d, s);
+  }
 
   // -- T1 right-hand side
   LAUNCH(r1init_vjp_kernel, ov, t1b, lam1, c->f, d);  // t1 (1-delta) f_ae
   {  // -1/2 tau_im^ef <ma||ef>, summed over m
     GemmDesc<double> g = G64(lam1, V, 1, c->ovvv, pv, 1, A->tauub, O * pv, 1, O, pv, V, -1.0, 1.0);
     g.nb1 = (int)O; g.sB1 = V * pv; g.sC1 = pv;
-    TRY(run_gemm(c, g, W));
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->lam1_f, L->ovvv32);
+      f.sBk = L->ld_pv; f.sB1 = V * L->ld_pv;
+      TRY(f32(f));
+    } else {
+      TRY(run_gemm(c, g, W));
+    }
     if (th) {
       GemmDesc<double> h = G64(lam1, 1, V, c->tauu, O * pv, 1, th->ovvv, pv, 1, V, pv, O, -1.0, 1.0);
       h.nb1 = (int)O; h.sB1 = pv; h.sC1 = V * pv;
@@ -421,13 +449,16 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
     }
   }
   // + W3 . Tv_flat
-  TRY(run_gemm(c, G64(lam1, V, 1, c->Tv, 1, V, A->W3b, po * V, 1, O, po * V, V, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->W3, 1, po * V, lam1, V, 1, A->Tvb, V, 1, po * V, V, O, 1.0, 1.0), W));
+  TRY(mm(G64(lam1, V, 1, c->Tv, 1, V, A->W3b, po * V, 1, O, po * V, V, 1.0, 1.0), L ? L->lam1_f : nullptr,
+         L ? L->Tv_f : nullptr));
+  TRY(mm(G64(c->W3, 1, po * V, lam1, V, 1, A->Tvb, V, 1, po * V, V, O, 1.0, 1.0), L ? L->W3_f : nullptr,
+         L ? L->lam1_f : nullptr));
   // + Tring . Fme
   LAUNCH(outer_add_kernel, ov * ov, A->Tringb, lam1, c->Fme, ov, ov, 1.0);
   TRY(run_gemm(c, G64(c->Tring, ov, 1, lam1, 1, 0, A->Fmeb, 1, 0, ov, 1, ov, 1.0, 1.0), W));
   // + G . t1   (G symmetric)
-  TRY(run_gemm(c, G64(c->G, ov, 1, lam1, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), W));
+  TRY(mm(G64(c->G, ov, 1, lam1, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), L ? L->G_f : nullptr,
+         L ? L->lam1_f : nullptr));
   if (th) LAUNCH(outer_add_kernel, ov * ov, th->G, lam1, t1, ov, ov, 1.0);
   // - Fmi^T . t1
   TRY(run_gemm(c, G64(t1, V, 1, lam1, 1, V, A->Fmib, O, 1, O, O, V, -1.0, 1.0), W));
@@ -439,20 +470,27 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   }
 
   // -- Z[i][j][ab] = -Fmi[m,j] T2u[i][m][ab] - t1[i,e] ovvv[(j,e)][ab]
+  const float* Zf = L ? L->Zb_f : nullptr;
   {
     GemmDesc<double> g = G64(c->T2u, pv, 1, A->Zb, 1, pv, A->Fmib, O, 1, O, O, pv, -1.0, 1.0);
     g.nb1 = (int)O; g.sA1 = O * pv; g.sB1 = O * pv; g.reduce_batch = true;
-    TRY(run_gemm(c, g, W));
+    TRY(mm(g, L ? L->T2u_f : nullptr, Zf));
   }
   {
     GemmDesc<double> g = G64(c->Fmi, O, 1, A->Zb, pv, 1, A->T2ub, pv, 1, O, pv, O, -1.0, 1.0);
     g.nb1 = (int)O; g.sB1 = O * pv; g.sC1 = O * pv;
-    TRY(run_gemm(c, g, W));
+    TRY(mm(g, L ? L->Fmi_f : nullptr, Zf));
   }
   {
     GemmDesc<double> g = G64(A->Zb, O * pv, 1, c->ovvv, 1, pv, t1b, V, 1, O, V, pv, -1.0, 1.0);
     g.nb1 = (int)O; g.sA1 = pv; g.sB1 = V * pv; g.reduce_batch = true;
-    TRY(run_gemm(c, g, W));
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, Zf, L->ovvv32);
+      f.sBn = L->ld_pv; f.sB1 = V * L->ld_pv;
+      TRY(f32(f));
+    } else {
+      TRY(run_gemm(c, g, W));
+    }
   }
   if (th) {
     GemmDesc<double> g = G64(t1, 1, V, A->Zb, O * pv, 1, th->ovvv, pv, 1, V, pv, O, -1.0, 1.0);
@@ -461,17 +499,18 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   }
 
   // -- Y[ij][a][b] = Tv[ij][a][e] Fae[b][e] - t1[m,a] W2[ij][(m,b)]
-  TRY(run_gemm(c, G64(A->Yb, V, 1, c->Fae, V, 1, A->Tvb, V, 1, po * V, V, V, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(A->Yb, 1, V, c->Tv, V, 1, A->Faeb, V, 1, V, V, po * V, 1.0, 1.0), W));
+  const float* Yf = L ? L->Yb_f : nullptr;
+  TRY(mm(G64(A->Yb, V, 1, c->Fae, V, 1, A->Tvb, V, 1, po * V, V, V, 1.0, 1.0), Yf, L ? L->Fae_f : nullptr));
+  TRY(mm(G64(A->Yb, 1, V, c->Tv, V, 1, A->Faeb, V, 1, V, V, po * V, 1.0, 1.0), Yf, L ? L->Tv_f : nullptr));
   {
     GemmDesc<double> g = G64(c->W2, V, 1, A->Yb, 1, V, t1b, V, 1, O, V, V, -1.0, 1.0);
     g.nb1 = (int)po; g.sA1 = ov; g.sB1 = VV; g.reduce_batch = true;
-    TRY(run_gemm(c, g, W));
+    TRY(mm(g, L ? L->W2_f : nullptr, Yf));
   }
   {
     GemmDesc<double> g = G64(t1, V, 1, A->Yb, V, 1, A->W2b, V, 1, O, V, V, -1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = VV; g.sC1 = ov;
-    TRY(run_gemm(c, g, W));
+    TRY(mm(g, L ? L->t1_f : nullptr, Yf));
   }
 
   // -- ring X = Tring . WT^T
@@ -485,14 +524,34 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   // -- Rp = <ab||ij> + tau . vvvv + W1^T . tau     (cotangent lam2)
   if (lowp) TRY(lowprec_adj_ladder(c, A->taub, lam2));
   else TRY(run_gemm(c, G64(lam2, pv, 1, c->vvvv, pv, 1, A->taub, pv, 1, po, pv, pv, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->tau, pv, 1, lam2, 1, pv, A->W1b, po, 1, po, po, pv, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->W1, po, 1, lam2, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), W));
+  {
+    GemmDesc<double> g = G64(c->tau, pv, 1, lam2, 1, pv, A->W1b, po, 1, po, po, pv, 1.0, 1.0);
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->tau32, L->lam2_f);
+      f.sAm = L->ld_pv; f.sBn = L->ld_pv;
+      TRY(f32(f));
+    } else {
+      TRY(run_gemm(c, g, W));
+    }
+  }
+  {
+    GemmDesc<double> g = G64(c->W1, po, 1, lam2, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0);
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, L->W1_f, L->lam2_f);
+      f.sBk = L->ld_pv;
+      TRY(f32(f));
+    } else {
+      TRY(run_gemm(c, g, W));
+    }
+  }
   if (th) LAUNCH(scale_add_kernel, po * pv, th->oovv, lam2, 1.0, po * pv);
 
   // -- F_ae = (1-d) f + Fae_t1 - Fme^T t1 - Tv_flat^T oovv_vu_flat
   TRY(run_gemm(c, G64(t1, V, 1, A->Faeb, V, 1, A->Fmeb, V, 1, O, V, V, -1.0, 1.0), W));
   TRY(run_gemm(c, G64(c->Fme, V, 1, A->Faeb, 1, V, t1b, V, 1, O, V, V, -1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->oovv_vu, V, 1, A->Faeb, 1, V, A->Tvb, V, 1, po * V, V, V, -1.0, 1.0), W));
+  if (lowp) k_to_f32(L->Faeb_f, A->Faeb, VV, s);  // F_ae's cotangent is complete here
+  TRY(mm(G64(c->oovv_vu, V, 1, A->Faeb, 1, V, A->Tvb, V, 1, po * V, V, V, -1.0, 1.0), L ? L->oovv_vu_f : nullptr,
+         L ? L->Faeb_f : nullptr));
   if (th) {
     TRY(run_gemm(c, G64(c->Tv, V, 1, A->Faeb, V, 1, th->oovv_vu, V, 1, po * V, V, V, -1.0, 1.0), W));
     LAUNCH(scale_add_kernel, VV, th->fvv, A->Faeb, 1.0, VV);
@@ -501,7 +560,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   // -- F_mi = (1-d) f + t1 <mn||ie> + oovv_ou . T2u^T + Fme . t1^T
   TRY(run_gemm(c, G64(A->Fmib, O, 1, t1, V, 1, A->Fmeb, V, 1, O, V, O, 1.0, 1.0), W));
   TRY(run_gemm(c, G64(A->Fmib, 1, O, c->Fme, V, 1, t1b, V, 1, O, V, O, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(A->Fmib, 1, O, c->oovv_ou, O * pv, 1, A->T2ub, O * pv, 1, O, O * pv, O, 1.0, 1.0), W));
+  if (lowp) k_to_f32(L->Fmib_f, A->Fmib, O * O, s);  // complete here
+  TRY(mm(G64(A->Fmib, 1, O, c->oovv_ou, O * pv, 1, A->T2ub, O * pv, 1, O, O * pv, O, 1.0, 1.0),
+         L ? L->Fmib_f : nullptr, L ? L->oovv_ou_f : nullptr));
   hipLaunchKernelGGL(fmi_t1_vjp_kernel, dim3(ov), dim3(kThreads), 0, s, t1b, A->Fmib, c->ooov, d);
   if (th) {
     TRY(run_gemm(c, G64(A->Fmib, O, 1, c->T2u, O * pv, 1, th->oovv_ou, O * pv, 1, O, O * pv, O, 1.0, 1.0), W));
@@ -528,18 +589,27 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   if (th) LAUNCH(scale_add_kernel, ov * ov, th->G, A->WTb, 1.0, ov * ov);  // + <mb||ej>
   // (a) S[(m,b,e)][j] = ovvv_u[(m,b,e)][f] t1[j][f]  (S_bar in Xb: Xb is free after the ring)
   double* Sb = A->Xb;
-  LAUNCH(sbar_kernel, O * VV * O, Sb, A->WTb, A->Faeb, d);
-  TRY(run_gemm(c, G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), W));
+  if (lowp) {
+    LAUNCH(sbar_kernel<float>, O * VV * O, L->Sb_f, A->WTb, A->Faeb, d);
+    TRY(f32(f32_desc(G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), L->Sb_f, L->ovvv_u_f)));
+  } else {
+    LAUNCH(sbar_kernel<double>, O * VV * O, Sb, A->WTb, A->Faeb, d);
+    TRY(run_gemm(c, G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), W));
+  }
   if (th) TRY(run_gemm(c, G64(Sb, O, 1, t1, V, 1, th->ovvv_u, V, 1, O * VV, V, O, 1.0, 1.0), W));
 
   // -- W_mbij: W2 = ooov + tau . ovvv^T + P(ij) U,  U[i][j][(m,b)] = t1[i,e] G[(m,e),(j,b)]
-  hipLaunchKernelGGL(unpack_pair_rows_kernel, dim3(grid_for(O * O * ov)), dim3(kThreads), 0, s, A->Ub, A->W2b, (int)O,
-                     ov);
+  if (lowp)
+    hipLaunchKernelGGL(unpack_pair_rows_kernel<float>, dim3(grid_for(O * O * ov)), dim3(kThreads), 0, s, L->Ub_f,
+                       A->W2b, (int)O, ov);
+  else
+    hipLaunchKernelGGL(unpack_pair_rows_kernel<double>, dim3(grid_for(O * O * ov)), dim3(kThreads), 0, s, A->Ub,
+                       A->W2b, (int)O, ov);
   {
     GemmDesc<double> g = G64(A->Ub, O * ov, 1, c->G, 1, ov, t1b, V, 1, O, V, V, 1.0, 1.0);
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = m, b2 = j
     g.sA1 = V; g.sA2 = ov; g.sB1 = V * ov; g.sB2 = V; g.reduce_batch = true;
-    TRY(run_gemm(c, g, W));
+    TRY(mm(g, L ? L->Ub_f : nullptr, L ? L->G_f : nullptr));
   }
   if (th) {
     GemmDesc<double> g = G64(t1, 1, V, A->Ub, O * ov, 1, th->G, ov, 1, V, V, O, 1.0, 1.0);
@@ -557,7 +627,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   // -- W_mnie: W3[i][mn][e] = ooov[mn][i,e] + t1[i,f] oovv_vu[mn][f][e];  W3u = unpack(W3)
   hipLaunchKernelGGL(pack_pair_rows_adj_kernel, dim3(grid_for(O * po * V)), dim3(kThreads), 0, s, A->W3b, A->W3ub,
                      (int)O, V, O);
-  TRY(run_gemm(c, G64(A->W3b, po * V, 1, c->oovv_vu, V, 1, t1b, V, 1, O, V, po * V, -1.0, 1.0), W));
+  if (lowp) k_to_f32(L->W3b_f, A->W3b, O * po * V, s);  // complete here
+  TRY(mm(G64(A->W3b, po * V, 1, c->oovv_vu, V, 1, t1b, V, 1, O, V, po * V, -1.0, 1.0), L ? L->W3b_f : nullptr,
+         L ? L->oovv_vu_f : nullptr));
   if (th) {
     GemmDesc<double> g = G64(t1, 1, V, A->W3b, po * V, 1, th->oovv_vu, V, 1, V, V, O, 1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = V; g.sC1 = VV;
@@ -569,7 +641,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   hipLaunchKernelGGL(unpack_pair_cols_kernel, dim3(grid_for(po * O * O)), dim3(kThreads), 0, s, A->Qb, A->W1b, po,
                      (int)O);
   TRY(run_gemm(c, G64(A->Qb, 1, O, c->ooov, V, 1, t1b, V, 1, O, V, po * O, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(A->W1b, 1, po, c->oovv, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), W));
+  if (lowp) k_to_f32(L->W1b_f, A->W1b, po * po, s);  // complete here
+  TRY(mm(G64(A->W1b, 1, po, c->oovv, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), L ? L->W1b_f : nullptr,
+         L ? L->oovv_f : nullptr));
   if (th) {
     TRY(run_gemm(c, G64(A->Qb, O, 1, t1, V, 1, th->ooov, V, 1, po * O, V, O, 1.0, 1.0), W));
     TRY(run_gemm(c, G64(A->W1b, po, 1, c->tau, pv, 1, th->oovv, pv, 1, po, pv, po, 1.0, 1.0), W));
@@ -631,7 +705,10 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   std::vector<double> B((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
   int count = 0, next = 0, it = 0, lowp_its = 0, fp64_after = c->opt.final_fp64_sweeps;
   double delta = 1e300;
-  const double tol = 1e-10;
+  // Lambda tolerance: the option, else 1e-2 x the forward's (clamped): the
+  // gradient's error is then set by where the amplitudes stopped
+  const double tol = c->opt.lambda_tol > 0.0 ? c->opt.lambda_tol
+                                              : std::min(1e-8, std::max(1e-11, 1e-2 * c->last_tol));
   bool conv = false, lowp = c->opt.precision == CCSD_PREC_BF16X3;
   double best_low = HUGE_VAL;  // smallest max|dLambda| of the BF16x3 phase
   int since_best = 0;
@@ -674,8 +751,8 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
     if (!extrap) std::swap(A->lam, A->lnew);
     if (lowp) {
       // refine in FP64 from here on; the DIIS subspace is kept (at O=40/V=300:
-      // 15 + 5 iterations against 15 + 7 with a reset), since the tight
-      // max|dLambda| < 1e-10 test on FP64 updates pins the FP64 fixed point
+      // 15 + 5 iterations against 15 + 7 with a reset), since the
+      // max|dLambda| < tol test on FP64 updates pins the FP64 fixed point
       const double f = c->opt.lambda_switch_factor > 0 ? c->opt.lambda_switch_factor : c->opt.switch_factor;
       if (delta < best_low) {
         best_low = delta;

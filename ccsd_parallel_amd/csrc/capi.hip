@@ -85,6 +85,7 @@ int init_device(Ctx* c, const ccsd_options* o, int O, int V) {
   if (o) c->opt = *o;
   if (c->opt.diis_dim < 0 || c->opt.diis_dim > 64 || c->opt.final_fp64_sweeps < 0) return CCSD_ERR_ARG;
   if (c->opt.precision < CCSD_PREC_FP64 || c->opt.precision > CCSD_PREC_BF16X3) return CCSD_ERR_ARG;
+  if (!(c->opt.lambda_tol >= 0.0)) return CCSD_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCSD_ERR_NO_DEVICE;
   if (c->opt.device < 0 || c->opt.device >= ndev) return CCSD_ERR_ARG;
@@ -252,7 +253,11 @@ int ccsd_default_options(ccsd_options* o) {
   // Switch at |dE| < 100 tol: at O=40/V=300 (tol 1e-7) that is 7 BF16x3
   // iterations and the 3 FP64 sweeps, against 10 + 3 when switching at tol.
   o->switch_factor = 100.0;
-  o->lambda_switch_factor = 0.0;
+  // Lambda iterations: switch to FP64 at max|dLambda| < 30 lambda_tol.  Measured at
+  // O=40/V=300 (forward to 1e-7, lambda_tol 1e-9): factor 30 gives 15 BF16x3 + 4 FP64
+  // iterations (2.64 s backward), 100 gives 13 + 6 (2.84 s), 300 gives 12 + 6 (2.92 s)
+  o->lambda_switch_factor = 30.0;
+  o->lambda_tol = 0.0;
   return CCSD_OK;
 }
 
@@ -481,6 +486,7 @@ int ccsd_solve(ccsd_ctx* h, double tol, int max_iter, ccsd_result* out) {
   r.seconds_per_iter = it ? r.seconds_total / it : 0.0;
   c->e_last = E;
   c->solved = converged;
+  c->last_tol = tol;
   if (out) *out = r;
   if (c->opt.verbose) {
     printf("CCSD correction [au] = %.12f\n", E);

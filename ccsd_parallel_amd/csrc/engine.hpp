@@ -60,6 +60,7 @@ struct Ctx {
   // state
   bool solved = false;
   double e_mp2 = 0.0, e_last = 0.0;
+  double last_tol = 1e-10;  // tol of the last ccsd_solve (sets the default Lambda tolerance)
   std::vector<double> energies;
   double timings_ms[7] = {0, 0, 0, 0, 0, 0, 0};  // [5], [6]: RHF set-up (transform, packing)
   hipEvent_t ev[12] = {};

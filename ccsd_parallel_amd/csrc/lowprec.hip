@@ -751,13 +751,37 @@ int lowprec_adj_setup(Ctx* c) {
   TRY(lowprec_allocate(c, CCSD_PREC_BF16X3));
   LowPrec* L = c->lp;
   const Dims& d = c->d;
+  const long O = d.O, V = d.V;
   hipStream_t s = c->stream;
   if (!L->have_adj) {
     TRY(lp_alloc(c, &L->WTt_b, 2 * d.ov * L->ld_ov));
     TRY(lp_alloc(c, &L->WTbar_b, 2 * d.ov * L->ld_ov));
     TRY(lp_alloc(c, &L->W2bar_b, 2 * d.po * L->ld_ov));
+    TRY(lp_alloc(c, &L->lam1_f, d.ov));
+    TRY(lp_alloc(c, &L->lam2_f, d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->Yb_f, d.po * V * V));
+    TRY(lp_alloc(c, &L->Zb_f, O * O * d.pv));
+    TRY(lp_alloc(c, &L->Faeb_f, V * V));
+    TRY(lp_alloc(c, &L->W1b_f, d.po * d.po));
+    TRY(lp_alloc(c, &L->Sb_f, O * V * V * O));
+    TRY(lp_alloc(c, &L->Ub_f, O * O * d.ov));
+    TRY(lp_alloc(c, &L->W3b_f, O * d.po * V));
+    TRY(lp_alloc(c, &L->Fmib_f, O * O));
     L->have_adj = 1;
   }
+  // FP32 copies of the forward intermediates at the converged amplitudes (the
+  // FP64 right-hand side just computed), for the secondary reverse products
+  const double* t1 = c->amp;
+  k_to_f32(L->t1_f, t1, d.ov, s);
+  k_to_f32(L->T2u_f, c->T2u, O * O * d.pv, s);
+  k_to_f32(L->Tv_f, c->Tv, d.po * V * V, s);
+  k_to_f32(L->W1_f, c->W1, d.po * d.po, s);
+  k_to_f32(L->W3_f, c->W3, O * d.po * V, s);
+  k_to_f32(L->W2_f, c->W2, d.po * d.ov, s);
+  k_to_f32(L->Fmi_f, c->Fmi, O * O, s);
+  k_to_f32(L->Fae_f, c->Fae, V * V, s);
+  hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.po * L->ld_pv)), dim3(kThreads), 0, s, L->tau32, c->tau, d.po,
+                     d.pv, d.pv, L->ld_pv, 1.0);
   if (!L->have_ovvvT) {  // <ia||bc>^T: rows (b<c), k = (i,a); static
     TRY(lp_alloc(c, &L->ovvvT_b, 2 * d.pv * L->ld_ov));
     hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.pv * L->ld_ov / 8)), dim3(kThreads), 0, s, L->ovvvT_b,
@@ -768,6 +792,20 @@ int lowprec_adj_setup(Ctx* c) {
   hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.ov * L->ld_ov / 8)), dim3(kThreads), 0, s, L->WTt_b,
                      c->WT, d.ov, d.ov, d.ov, L->ld_ov);
   conv_il(c, L->Tring_b, c->Tring, d.ov, d.ov, L->ld_ov);
+  HIPTRY(hipGetLastError());
+  return CCSD_OK;
+}
+
+int lowprec_adj_lambda(Ctx* c, const double* lam) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  const double* lam2 = lam + d.ov;
+  k_to_f32(L->lam1_f, lam, d.ov, s);
+  hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.po * L->ld_pv)), dim3(kThreads), 0, s, L->lam2_f, lam2, d.po,
+                     d.pv, d.pv, L->ld_pv, 1.0);
+  k_unpack_vir_ring_lp(L->Yb_f, RingOut{}, lam2, d, s);
+  k_unpack_occ_f32(L->Zb_f, L->Zb_f, lam2, lam2, d, s);
   HIPTRY(hipGetLastError());
   return CCSD_OK;
 }

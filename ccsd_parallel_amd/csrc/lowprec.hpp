@@ -38,6 +38,13 @@ struct LowPrec {
   // W_mbij cotangents
   uint16_t *WTt_b = nullptr, *ovvvT_b = nullptr, *WTbar_b = nullptr, *W2bar_b = nullptr;
   int have_adj = 0, have_ovvvT = 0;
+  // FP32 cotangents of the low-precision Lambda iterations' secondary products
+  // (the forward intermediates they pair with reuse the FP32 copies above,
+  // refreshed at the converged amplitudes by lowprec_adj_setup): lam1, lam2
+  // (rows padded to ld_pv, as tau32), the unpacked lam2 (Yb, Zb), and the
+  // small cotangents converted once complete
+  float *lam1_f = nullptr, *lam2_f = nullptr, *Yb_f = nullptr, *Zb_f = nullptr, *Faeb_f = nullptr;
+  float *W1b_f = nullptr, *Sb_f = nullptr, *Ub_f = nullptr, *W3b_f = nullptr, *Fmib_f = nullptr;
   std::vector<void*> allocs;
 };
 
@@ -65,6 +72,9 @@ int lowprec_ring(Ctx* c, int precision);
 //   wmbej:  Tringb += 1/2 WTb Doovv
 //   w2:     taub += W2b <mb||ef>
 int lowprec_adj_setup(Ctx* c);
+// lam -> lam1_f, lam2_f and the FP32 unpacks Yb_f (virtual pairs) and Zb_f
+// (occupied pairs) for one Lambda iteration's secondary products.
+int lowprec_adj_lambda(Ctx* c, const double* lam);
 int lowprec_adj_ring(Ctx* c, double* Tringb, double* WTb, const double* lam2);
 int lowprec_adj_ladder(Ctx* c, double* taub, const double* lam2);
 int lowprec_adj_wmbej(Ctx* c, double* Tringb, const double* WTb);

@@ -79,9 +79,13 @@ typedef struct ccsd_options {
   double switch_factor;     /* mixed modes: switch once |dE| < switch_factor * tol (default 100: the FP64
                                sweeps then take |dE| down the last two decades) */
   double lambda_switch_factor; /* backward, BF16X3: switch the Lambda iterations to FP64 once
-                                  max|dLambda| < this * 1e-10 (0 = switch_factor), or once
+                                  max|dLambda| < this * lambda_tol (default 30; 0 = switch_factor), or once
                                   max|dLambda| has set no new minimum in 4 BF16x3 iterations
                                   (its rounding floor) */
+  double lambda_tol;        /* backward: the Lambda iterations stop at max|dLambda| < lambda_tol;
+                               0 (default) = 1e-2 x the tol of the last ccsd_solve, clamped to
+                               [1e-11, 1e-8] (the gradient is then as accurate as the amplitudes it
+                               is taken at) */
 } ccsd_options;
 
 typedef struct ccsd_problem {
