@@ -84,7 +84,27 @@ struct KArgs {
   // to tail_partial[unit]; a second kernel sums them in a fixed order.
   int tail_split, xcd_full, tail_units, tail_kchunk;
   T* tail_partial;
+  // Tile walk: 0 for column-major over the tiles_m x tiles_n grid; g > 0 walks
+  // bands of g tile rows, column-major inside a band.
+  int group;
 };
+
+// Tile id -> (tile row, tile column).  In bands of `group` tile rows a run of
+// consecutive ids (one XCD's slots at a time) covers a window about
+// group x (run / group) tiles instead of one tall column strip, so each
+// A and B panel it reads is shared by several resident workgroups
+// (12000^3 in 128 x 128 tiles: 3% faster with bands of 8 rows, measured).
+__device__ __forceinline__ void tile_coords(int tile, int tiles_m, int tiles_n, int group, int& tm, int& tn) {
+  if (group <= 0) {
+    tm = tile % tiles_m;
+    tn = tile / tiles_m;
+    return;
+  }
+  const int band = group * tiles_n, first = (tile / band) * group;
+  const int rows = min(tiles_m - first, group), r = tile % band;
+  tm = first + r % rows;
+  tn = r / rows;
+}
 
 // VWT: elements per vector load (2: 16-byte double pairs or 8-byte float
 // pairs, 4: 16-byte float quads), 0 for single-element loads (in units of 2).
@@ -148,8 +168,8 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void gemm_kernel(KArgs<T> p) {
     if (tile >= (int)(((long)(x + 1) * ntiles) >> 3)) return;
   }
   if (tile >= ntiles) return;
-  const int tm = tile % p.tiles_m;
-  const int tn = tile / p.tiles_m;
+  int tm, tn;
+  tile_coords(tile, p.tiles_m, p.tiles_n, p.group, tm, tn);
   const int b1 = batch / d.nb2, b2 = batch % d.nb2;
 
   const T* __restrict__ A = d.A + b1 * d.sA1 + b2 * d.sA2;
@@ -534,13 +554,14 @@ __global__ void gemv_kernel(GemmDesc<T> d) {
 // are dropped here.
 template <typename T>
 __global__ void gemm_tail_reduce_kernel(GemmDesc<T> d, const T* __restrict__ W, int first_tile, int tiles_m,
-                                        int ntail, int nsplit, int bm, int bn) {
+                                        int tiles_n, int group, int ntail, int nsplit, int bm, int bn) {
   const long per = (long)bm * bn;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < ntail * per;
        idx += (long)gridDim.x * blockDim.x) {
     const long tt = idx / per, e = idx % per;
-    const int tile = first_tile + (int)tt;
-    const int row = (tile % tiles_m) * bm + (int)(e / bn), col = (tile / tiles_m) * bn + (int)(e % bn);
+    int tm, tn;
+    tile_coords(first_tile + (int)tt, tiles_m, tiles_n, group, tm, tn);
+    const int row = tm * bm + (int)(e / bn), col = tn * bn + (int)(e % bn);
     if (row >= d.M || col >= d.N) continue;
     T s = 0;
     for (int sp = 0; sp < nsplit; ++sp) s += W[(tt * nsplit + sp) * per + e];
@@ -624,6 +645,7 @@ constexpr Cfg kCfgs[] = {
     {112, 128, 16, 1, 8, 2, 67.0f, true, true, true},
 };
 constexpr int kNumCus = 256;
+constexpr int kTileBand = 8;  // tile rows per band of the tile walk (tile_coords)
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, int WPE, int VWT>
@@ -838,6 +860,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
   ka.tail_units = (int)(tail * tail_split);
   ka.tail_kchunk = tail_kchunk;
   ka.tail_partial = tail_split ? (T*)ws.ptr : nullptr;
+  ka.group = tiles_m >= 2 * kTileBand && tiles_n >= kTileBand ? kTileBand : 0;
   if (ka.tiles_pad <= 0 || splits > 65535) return hipErrorInvalidValue;
   dim3 grid(ka.tiles_pad, nbatch, splits);
   if (tail_split) grid = dim3((unsigned)(8 * (xcd_full + (ka.tail_units + 7) / 8)), 1, 1);
@@ -847,7 +870,7 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     const long n = tail * g.BM * g.BN;
     const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(gemm_tail_reduce_kernel<T>, dim3(blocks), dim3(256), 0, stream, d, (const T*)ws.ptr,
-                       (int)(8 * xcd_full), tiles_m, (int)tail, tail_split, g.BM, g.BN);
+                       (int)(8 * xcd_full), tiles_m, tiles_n, ka.group, (int)tail, tail_split, g.BM, g.BN);
     e = hipGetLastError();
   }
   if (use_partial) {

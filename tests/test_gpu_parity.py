@@ -94,6 +94,24 @@ def test_gemm_split_tail(prec, M, N, K, ta, tb, tile):
     assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K) * max(1.0, np.abs(ref).max() / np.sqrt(K))
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("M,N,K,ta,tb", [(2100, 1000, 300, 0, 1), (1000, 2100, 257, 1, 0), (2050, 1030, 200, 0, 0)])
+def test_gemm_tile_bands(prec, M, N, K, ta, tb):
+    """128 x 128 tiles (forced) with at least 16 tile rows and 8 tile columns,
+    fewer tiles than one full round: the tile walk goes in bands of 8 tile rows
+    with a short last band (17 rows: 8 + 8 + 1) and no split tail."""
+    capi = _ctx()
+    rng = np.random.default_rng(3 * M + N + K + ta + 2 * tb)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 1.1 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.25 * C0
+    out = capi.selftest_gemm(A, B, C0, 1.1, 0.25, ta=bool(ta), tb=bool(tb), precision=prec, tile=3)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K) * max(1.0, np.abs(ref).max() / np.sqrt(K))
+
+
 @pytest.mark.parametrize("prec,tile,tol", [("fp32", t, 2e-5) for t in (1, 9, 10, 12)] +
                          [("bf16x3", t, 5e-5) for t in range(1, N_BF16_TILES + 1)])
 @pytest.mark.parametrize("M,N,K", [(300, 520, 200), (40, 45, 2000)])
