@@ -300,7 +300,9 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   k_copy(c->W1, c->oooo, po * po, s);
   {
     GemmDesc<double> g = G64(c->oovv, pv, 1, c->tau, 1, pv, c->W1, po, 1, po, po, pv, 1.0, 1.0);
-    if (lowp) {
+    if (prec == CCSD_PREC_BF16X3) {
+      TRY(lowprec_w1_bf16(c));
+    } else if (lowp) {
       GemmDesc<float> f = f32_desc(g, L->oovv_f, L->tau32);
       f.sBn = L->ld_pv;
       TRY(run_f32(c, f));
@@ -407,7 +409,9 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (time_it) HIPTRY(hipEventRecord(c->ev[6], s));
   {
     GemmDesc<double> g = G64(c->W1, 1, po, c->tau, pv, 1, c->Rp, pv, 1, po, pv, po, 1.0, 1.0);
-    if (lowp) {
+    if (prec == CCSD_PREC_BF16X3) {
+      TRY(lowprec_hh_bf16(c));
+    } else if (lowp) {
       GemmDesc<float> f = f32_desc(g, L->W1_f, L->tau32);
       f.sBk = L->ld_pv;
       TRY(run_f32(c, f));
@@ -426,8 +430,12 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (time_it) HIPTRY(hipEventRecord(c->ev[8], s));
 
   // -- Y[ij][a][b] = t_ij^ae F_be - t_m^a W_mbij
-  TRY(mm(G64(c->Tv, V, 1, c->Fae, 1, V, c->Y, V, 1, po * V, V, V, 1.0, 0.0), lowp ? L->Tv_f : nullptr,
-         lowp ? L->Fae_f : nullptr));
+  if (prec == CCSD_PREC_BF16X3) {
+    TRY(lowprec_y_bf16(c));
+  } else {
+    TRY(mm(G64(c->Tv, V, 1, c->Fae, 1, V, c->Y, V, 1, po * V, V, V, 1.0, 0.0), lowp ? L->Tv_f : nullptr,
+           lowp ? L->Fae_f : nullptr));
+  }
   {
     GemmDesc<double> g = G64(t1, 1, V, c->W2, V, 1, c->Y, V, 1, V, V, O, -1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = ov; g.sC1 = VV;

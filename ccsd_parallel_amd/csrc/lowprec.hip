@@ -43,7 +43,8 @@ __device__ __forceinline__ long il_index(long row, long k, long ld) { return row
 
 // dst (interleaved, rows x ld) = split(scale * src[r*sld + k]) for k < K, 0 beyond;
 // one thread per 8-element chunk (two 16-byte stores).
-__global__ void conv_bf16il_kernel(uint16_t* dst, const double* src, long rows, long K, long sld, long ld,
+template <typename S>
+__global__ void conv_bf16il_kernel(uint16_t* dst, const S* src, long rows, long K, long sld, long ld,
                                    double scale) {
   const long chunks = ld / 8, n = rows * chunks;
   GRID_LOOP(idx, n) {
@@ -54,7 +55,7 @@ __global__ void conv_bf16il_kernel(uint16_t* dst, const double* src, long rows, 
       const long k = c * 8 + e;
       h[e] = 0;
       l[e] = 0;
-      if (k < K) bf16_split(scale * src[r * sld + k], h[e], l[e]);
+      if (k < K) bf16_split(scale * (double)src[r * sld + k], h[e], l[e]);
     }
     uint4 hv, lv;
     hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
@@ -500,7 +501,12 @@ size_t lowprec_plan_bytes(const Dims& d, int precision) {
   // dominant-contraction operands: vvvv, ovvv, Doovv, tau, Tring, WT (fp32: 4 B; bf16 hi+lo: 2 x 2 B)
   long elems = d.pv * lpv + d.ov * lpv + d.ov * lov + d.po * lpv + 2 * d.ov * lov;
   size_t bytes = (size_t)elems * 4;
-  if (precision == CCSD_PREC_BF16X3) bytes += (size_t)(d.ov * lpv + d.po * lpv) * 4;  // FP32 ovvv, tau as well
+  if (precision == CCSD_PREC_BF16X3) {
+    bytes += (size_t)(d.ov * lpv + d.po * lpv) * 4;  // FP32 ovvv, tau as well
+    // BF16 operands of the W_mnij, hh-ladder and Y products: oovv, W1^T, tau^T, Tv, Fae
+    const long lpo = pad64(d.po), lv = pad64(V);
+    bytes += (size_t)(d.po * lpv + d.po * lpo + d.pv * lpo + d.po * V * lv + V * lv) * 4;
+  }
   // secondary FP32 copies: oovv, oovv_vu, oovv_ou, G, ovvv_u; t1, T2u, tauu, Tv, W1, W3, W3u, W2, Fae, Fmi
   long sec = d.po * d.pv + d.po * V * V + O * O * d.pv + d.ov * d.ov + d.ov * V * V;
   sec += d.ov + 2 * O * O * d.pv + d.po * V * V + d.po * d.po + 2 * O * d.po * V + O * O * O * V + d.po * d.ov +
@@ -537,7 +543,7 @@ static int lp_alloc(Ctx* c, T** p, long n) {
   } while (0)
 
 static void conv_il(Ctx* c, uint16_t* dst, const double* src, long rows, long K, long ld) {
-  hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(rows * ld / 8)), dim3(kThreads), 0, c->stream, dst, src, rows,
+  hipLaunchKernelGGL(conv_bf16il_kernel<double>, dim3(grid_for(rows * ld / 8)), dim3(kThreads), 0, c->stream, dst, src, rows,
                      K, K, ld, 1.0);
 }
 
@@ -554,6 +560,8 @@ int lowprec_allocate(Ctx* c, int precision) {
   const long O = d.O, V = d.V;
   L->ld_pv = pad64(d.pv);
   L->ld_ov = pad64(d.ov);
+  L->ld_po = pad64(d.po);
+  L->ld_v = pad64(V);
   hipStream_t s = c->stream;
   if (!L->have_sec) {
     // FP32 copies shared by both modes: ovvv and tau (k-contiguous, padded), the
@@ -604,12 +612,18 @@ int lowprec_allocate(Ctx* c, int precision) {
     TRY(lp_alloc(c, &L->tau_b, 2 * d.po * L->ld_pv));
     TRY(lp_alloc(c, &L->Tring_b, 2 * d.ov * L->ld_ov));
     TRY(lp_alloc(c, &L->WT_b, 2 * d.ov * L->ld_ov));
+    TRY(lp_alloc(c, &L->oovv_b, 2 * d.po * L->ld_pv));
+    TRY(lp_alloc(c, &L->W1T_b, 2 * d.po * L->ld_po));
+    TRY(lp_alloc(c, &L->tauT_b, 2 * d.pv * L->ld_po));
+    TRY(lp_alloc(c, &L->Tv_b, 2 * d.po * V * L->ld_v));
+    TRY(lp_alloc(c, &L->Fae_b, 2 * V * L->ld_v));
     // WT_b (W_mbej epilogue) and Tring_b (unpack) are written for columns < ov
     // only: zero the padding once
     for (uint16_t* q : {L->WT_b, L->Tring_b}) HIPTRY(hipMemsetAsync(q, 0, (size_t)d.ov * L->ld_ov * 4, s));
     conv_il(c, L->vvvv_b, c->vvvv, d.pv, d.pv, L->ld_pv);
     conv_il(c, L->ovvv_b, c->ovvv, d.ov, d.pv, L->ld_pv);
     conv_il(c, L->Doovv_b, c->Doovv, d.ov, d.ov, L->ld_ov);
+    conv_il(c, L->oovv_b, c->oovv, d.po, d.pv, L->ld_pv);
     L->have_bf16 = 1;
   }
   hipError_t e = hipGetLastError();
@@ -745,6 +759,43 @@ static int bf16_nt(Ctx* c, const uint16_t* A, long lda, const uint16_t* B, long 
     return CCSD_ERR_HIP;
   }
   return CCSD_OK;
+}
+
+// W1 += <mn||ef> tau_ij^ef (C[mn][ij]; W_1 of Fill_W1_and_W3 :659-1029): both
+// operands k-contiguous over the virtual pairs (tau_b is the ladder's operand).
+int lowprec_w1_bf16(Ctx* c) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  return bf16_nt(c, L->oovv_b, L->ld_pv, L->tau_b, L->ld_pv, d.po, d.po, d.pv, c->W1, d.po, 1.0, 1.0);
+}
+
+// Rp[ij][ab] += sum_mn W1[mn][ij] tau[mn][ab] (the hh ladder of Fill_t2_new
+// :2001-2458): as C = A B^T with A = W1^T and B = tau^T, both formed here.
+int lowprec_hh_bf16(Ctx* c) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.po * L->ld_po / 8)), dim3(kThreads), 0, s, L->W1T_b,
+                     c->W1, d.po, d.po, d.po, L->ld_po);
+  hipLaunchKernelGGL(conv_bf16il_t_kernel, dim3(grid_for(d.pv * L->ld_po / 8)), dim3(kThreads), 0, s, L->tauT_b,
+                     c->tau, d.pv, d.po, d.pv, L->ld_po);
+  HIPTRY(hipGetLastError());
+  return bf16_nt(c, L->W1T_b, L->ld_po, L->tauT_b, L->ld_po, d.po, d.pv, d.po, c->Rp, d.pv, 1.0, 1.0);
+}
+
+// Y[ij][a][b] = sum_e t_ij^ae F_be (Fill_t2_new's P(ab) F_ae term): rows (ij, a)
+// of Tv against rows b of F_ae, both k-contiguous over e.
+int lowprec_y_bf16(Ctx* c) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  const long V = d.V;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(conv_bf16il_kernel<float>, dim3(grid_for(d.po * V * L->ld_v / 8)), dim3(kThreads), 0, s,
+                     L->Tv_b, L->Tv_f, d.po * V, V, V, L->ld_v, 1.0);
+  hipLaunchKernelGGL(conv_bf16il_kernel<double>, dim3(grid_for(V * L->ld_v / 8)), dim3(kThreads), 0, s, L->Fae_b,
+                     c->Fae, V, V, V, L->ld_v, 1.0);
+  HIPTRY(hipGetLastError());
+  return bf16_nt(c, L->Tv_b, L->ld_v, L->Fae_b, L->ld_v, d.po * V, V, V, c->Y, V, 1.0, 0.0);
 }
 
 int lowprec_adj_setup(Ctx* c) {
@@ -893,9 +944,9 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
       GemmWorkspace ws{wsp, kWs};
       e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
     } else {
-      hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(M * ld / 8)), dim3(kThreads), 0, s, Ab, A, (long)M,
+      hipLaunchKernelGGL(conv_bf16il_kernel<double>, dim3(grid_for(M * ld / 8)), dim3(kThreads), 0, s, Ab, A, (long)M,
                          (long)K, (long)K, ld, 1.0);
-      hipLaunchKernelGGL(conv_bf16il_kernel, dim3(grid_for(N * ld / 8)), dim3(kThreads), 0, s, Bb, B, (long)N,
+      hipLaunchKernelGGL(conv_bf16il_kernel<double>, dim3(grid_for(N * ld / 8)), dim3(kThreads), 0, s, Bb, B, (long)N,
                          (long)K, (long)K, ld, 1.0);
       Bf16Out o;
       o.C64 = C; o.ldc = N; o.alpha = alpha; o.beta = beta;

@@ -20,6 +20,7 @@ namespace ccsd {
 struct LowPrec {
   int have_fp32 = 0, have_bf16 = 0, have_sec = 0;
   long ld_pv = 0, ld_ov = 0;  // padded leading dims for K = pv and K = ov
+  long ld_po = 0, ld_v = 0;   // ... and for K = po and K = V
   // FP32 operands of the dominant contractions (k-contiguous, padded); ovvv32
   // and tau32 also serve the secondary contractions in both modes
   float *vvvv32 = nullptr, *ovvv32 = nullptr, *Doovv32 = nullptr;
@@ -27,6 +28,9 @@ struct LowPrec {
   // BF16 hi/lo operands, interleaved per 8-element chunk (row r at r * 2 * ld)
   uint16_t *vvvv_b = nullptr, *ovvv_b = nullptr, *Doovv_b = nullptr;
   uint16_t *tau_b = nullptr, *Tring_b = nullptr, *WT_b = nullptr;
+  // BF16 hi/lo operands of three secondary products of the BF16x3 iterations:
+  // <mn||ef> (static), W1^T and tau^T (hh ladder), Tv and F_ae (Y)
+  uint16_t *oovv_b = nullptr, *W1T_b = nullptr, *tauT_b = nullptr, *Tv_b = nullptr, *Fae_b = nullptr;
   // FP32 copies for the secondary contractions, same layouts as the FP64 arrays:
   // static integrals ...
   float *oovv_f = nullptr, *oovv_vu_f = nullptr, *oovv_ou_f = nullptr, *G_f = nullptr, *ovvv_u_f = nullptr;
@@ -62,6 +66,12 @@ int lowprec_w2(Ctx* c, int precision);
 int lowprec_wmbej(Ctx* c, int precision);
 int lowprec_ladder(Ctx* c, int precision);
 int lowprec_ring(Ctx* c, int precision);
+// BF16x3 forms of three secondary products (BF16x3 iterations only):
+//   W1 += <mn||ef> tau^T;  Rp += W1^T tau (hh ladder, W1 complete);
+//   Y = Tv F_ae^T (F_ae complete, Tv_f current)
+int lowprec_w1_bf16(Ctx* c);
+int lowprec_hh_bf16(Ctx* c);
+int lowprec_y_bf16(Ctx* c);
 
 // Low-precision (BF16x3) pieces of the adjoint's reverse pass, used by the
 // Lambda iterations before their FP64 sweeps.  lowprec_adj_setup converts the

@@ -94,7 +94,6 @@ def test_gemm_split_tail(prec, M, N, K, ta, tb, tile):
     assert np.max(np.abs(out - ref)) <= tol * np.sqrt(K) * max(1.0, np.abs(ref).max() / np.sqrt(K))
 
 
-@pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("M,N,K,ta,tb", [(2100, 1000, 300, 0, 1), (1000, 2100, 257, 1, 0), (2050, 1030, 200, 0, 0)])
 def test_gemm_tile_bands(prec, M, N, K, ta, tb):
@@ -344,7 +343,10 @@ def test_low_precision_fixed_point_is_near_fp64(prec, floor):
     assert abs(tr[-1] - e64) <= floor * max(1.0, abs(e64))
 
 
-@pytest.mark.parametrize("O,V,seed", [(8, 36, 77), (3, 5, 78)])  # (3, 5): odd counts, every tile mostly clamped rows
+# (3, 5): odd counts, every tile mostly clamped rows; (50, 20): more occupied
+# than virtual orbitals, N = 50 in the batched W_mbej t1 product; (2, 330):
+# long packed rows (P2(V) = 54285) with a single occupied pair
+@pytest.mark.parametrize("O,V,seed", [(8, 36, 77), (3, 5, 78), (50, 20, 80), (2, 330, 81)])
 @pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 5e-4)])
 def test_low_precision_update_matches_fp64_update(prec, tol, O, V, seed):
     """One amplitude update from the same amplitudes (T1 nonzero everywhere) in
