@@ -354,10 +354,14 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     GemmDesc<double> g = G64(c->ovvv_u, V, 1, t1, 1, V, S, O, 1, O * VV, O, V, 1.0, 0.0);
     if (lowp) {  // S in FP32 (half the traffic of the two consumers)
       float* S32 = reinterpret_cast<float*>(c->X);
-      GemmDesc<float> f = f32_desc(g, L->ovvv_u_f, t1f);
-      f.Cd = nullptr;
-      f.C = S32;
-      TRY(run_f32(c, f));
+      if (prec == CCSD_PREC_BF16X3) {
+        TRY(lowprec_wmbej_s_bf16(c, t1, S32));
+      } else {
+        GemmDesc<float> f = f32_desc(g, L->ovvv_u_f, t1f);
+        f.Cd = nullptr;
+        f.C = S32;
+        TRY(run_f32(c, f));
+      }
       k_fae_t1_from_s_f32(c->Fae_t1, S32, d, s);
       k_wt_from_s_f32(c->WT, S32, L->G_f, d, s);
     } else {

@@ -455,8 +455,10 @@ hipError_t launch_bf16(Bf16Args p, hipStream_t s, const GemmWorkspace* ws) {
 // M = P2(O) = 780 (5 x 160 = 800 rows instead of 4 x 256 = 1024), 128 x 128 on
 // 4 waves for small ones.  The grouped tile order pays on square products with
 // many tile rows (measured ~3% on 12000^3) and not on the P2(O)-row ones.
+// 256 x 48 on 4 waves (wave tile 64 x 48) serves N <= 64: the t1 product of
+// W_mbej has N = O = 40 (83% of a 48-wide tile against 31% of a 128-wide one).
 // force (tests): 0 = 128 x 128, 1 = 256 x 256 automatic order, 2 = 256 x 256
-// plain order, 3 = 256 x 256 grouped order, 4 = 160 x 320.
+// plain order, 3 = 256 x 256 grouped order, 4 = 160 x 320, 5 = 256 x 48.
 hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, const Bf16Out& out, int M, int N,
                      int K, hipStream_t s, int force = -1, const GemmWorkspace* ws = nullptr) {
   if (M <= 0 || N <= 0) return hipSuccess;
@@ -468,10 +470,11 @@ hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, c
   p.M = M; p.N = N; p.Kp = ((K + 31) / 32) * 32;
   p.group = 0;
   struct Opt { int BM, BN; double rate; };
-  const Opt opts[3] = {{128, 128, 1.0}, {256, 256, 1.65}, {160, 320, 1.55}};
+  const Opt opts[4] = {{128, 128, 1.0}, {256, 256, 1.65}, {160, 320, 1.55}, {256, 48, 1.45}};
   int best = 0;
   double bt = 1e300;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < 4; ++c) {
+    if (c == 3 && N > 64) continue;  // 256 x 48: for N of one occupied-orbital count
     long tm = (M + opts[c].BM - 1) / opts[c].BM, tn = (N + opts[c].BN - 1) / opts[c].BN;
     double pad = double(tm * opts[c].BM) * double(tn * opts[c].BN) / (double(M) * N);
     double tiles = double(tm * tn);
@@ -487,8 +490,10 @@ hipError_t bf16x3_nt(const uint16_t* A, long lda, const uint16_t* B, long ldb, c
     if (force == 3) group = 8;
   }
   if (force == 4) best = 2;
+  if (force == 5) best = 3;
   if (best == 0) return launch_bf16<128, 128, 2, 2>(p, s, ws);
   if (best == 2) return launch_bf16<160, 320, 2, 4>(p, s, ws);
+  if (best == 3) return launch_bf16<256, 48, 4, 1>(p, s, ws);
   p.group = group;
   return launch_bf16<256, 256, 4, 2>(p, s, ws);
 }
@@ -503,9 +508,10 @@ size_t lowprec_plan_bytes(const Dims& d, int precision) {
   size_t bytes = (size_t)elems * 4;
   if (precision == CCSD_PREC_BF16X3) {
     bytes += (size_t)(d.ov * lpv + d.po * lpv) * 4;  // FP32 ovvv, tau as well
-    // BF16 operands of the W_mnij, hh-ladder and Y products: oovv, W1^T, tau^T, Tv, Fae
+    // BF16 operands of the W_mnij, hh-ladder, Y and W_mbej t1 products: oovv,
+    // W1^T, tau^T, Tv, Fae, the unpacked <mb||ef> and t1
     const long lpo = pad64(d.po), lv = pad64(V);
-    bytes += (size_t)(d.po * lpv + d.po * lpo + d.pv * lpo + d.po * V * lv + V * lv) * 4;
+    bytes += (size_t)(d.po * lpv + d.po * lpo + d.pv * lpo + d.po * V * lv + V * lv + d.ov * V * lv + O * lv) * 4;
   }
   // secondary FP32 copies: oovv, oovv_vu, oovv_ou, G, ovvv_u; t1, T2u, tauu, Tv, W1, W3, W3u, W2, Fae, Fmi
   long sec = d.po * d.pv + d.po * V * V + O * O * d.pv + d.ov * d.ov + d.ov * V * V;
@@ -617,6 +623,8 @@ int lowprec_allocate(Ctx* c, int precision) {
     TRY(lp_alloc(c, &L->tauT_b, 2 * d.pv * L->ld_po));
     TRY(lp_alloc(c, &L->Tv_b, 2 * d.po * V * L->ld_v));
     TRY(lp_alloc(c, &L->Fae_b, 2 * V * L->ld_v));
+    TRY(lp_alloc(c, &L->ovvv_u_b, 2 * d.ov * V * L->ld_v));
+    TRY(lp_alloc(c, &L->t1_b, 2 * O * L->ld_v));
     // WT_b (W_mbej epilogue) and Tring_b (unpack) are written for columns < ov
     // only: zero the padding once
     for (uint16_t* q : {L->WT_b, L->Tring_b}) HIPTRY(hipMemsetAsync(q, 0, (size_t)d.ov * L->ld_ov * 4, s));
@@ -624,6 +632,7 @@ int lowprec_allocate(Ctx* c, int precision) {
     conv_il(c, L->ovvv_b, c->ovvv, d.ov, d.pv, L->ld_pv);
     conv_il(c, L->Doovv_b, c->Doovv, d.ov, d.ov, L->ld_ov);
     conv_il(c, L->oovv_b, c->oovv, d.po, d.pv, L->ld_pv);
+    conv_il(c, L->ovvv_u_b, c->ovvv_u, d.ov * V, V, L->ld_v);
     L->have_bf16 = 1;
   }
   hipError_t e = hipGetLastError();
@@ -796,6 +805,29 @@ int lowprec_y_bf16(Ctx* c) {
                      c->Fae, V, V, V, L->ld_v, 1.0);
   HIPTRY(hipGetLastError());
   return bf16_nt(c, L->Tv_b, L->ld_v, L->Fae_b, L->ld_v, d.po * V, V, V, c->Y, V, 1.0, 0.0);
+}
+
+// S[(m,b,e)][j] = sum_f <mb||ef> t_j^f into FP32 S32 (rows of O), the t1 term
+// of W_mbej (W_4, Fill_W4_MPI :4781-5089): rows of the unpacked block against
+// the rows of t1, both k-contiguous over f, on the 256 x 48 tile.
+int lowprec_wmbej_s_bf16(Ctx* c, const double* t1, float* S32) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  const long V = d.V;
+  conv_il(c, L->t1_b, t1, d.O, V, L->ld_v);
+  HIPTRY(hipGetLastError());
+  Bf16Out o;
+  o.C = S32;
+  o.ldc = d.O;
+  o.alpha = 1.0;
+  o.beta = 0.0;
+  hipError_t e = bf16x3_nt(L->ovvv_u_b, L->ld_v, L->t1_b, L->ld_v, o, (int)(d.ov * V), (int)d.O, (int)V, c->stream,
+                           -1, &c->ws);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  return CCSD_OK;
 }
 
 int lowprec_adj_setup(Ctx* c) {

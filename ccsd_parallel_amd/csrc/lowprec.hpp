@@ -31,6 +31,8 @@ struct LowPrec {
   // BF16 hi/lo operands of three secondary products of the BF16x3 iterations:
   // <mn||ef> (static), W1^T and tau^T (hh ladder), Tv and F_ae (Y)
   uint16_t *oovv_b = nullptr, *W1T_b = nullptr, *tauT_b = nullptr, *Tv_b = nullptr, *Fae_b = nullptr;
+  // ... and of W_mbej's t1 term: the unpacked <mb||ef> (static) and t1
+  uint16_t *ovvv_u_b = nullptr, *t1_b = nullptr;
   // FP32 copies for the secondary contractions, same layouts as the FP64 arrays:
   // static integrals ...
   float *oovv_f = nullptr, *oovv_vu_f = nullptr, *oovv_ou_f = nullptr, *G_f = nullptr, *ovvv_u_f = nullptr;
@@ -72,6 +74,8 @@ int lowprec_ring(Ctx* c, int precision);
 int lowprec_w1_bf16(Ctx* c);
 int lowprec_hh_bf16(Ctx* c);
 int lowprec_y_bf16(Ctx* c);
+// S[(m,b,e)][j] = sum_f <mb||ef> t_j^f into FP32 S32 [(m,b,e)][j] (BF16x3)
+int lowprec_wmbej_s_bf16(Ctx* c, const double* t1, float* S32);
 
 // Low-precision (BF16x3) pieces of the adjoint's reverse pass, used by the
 // Lambda iterations before their FP64 sweeps.  lowprec_adj_setup converts the

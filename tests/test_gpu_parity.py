@@ -53,7 +53,7 @@ def test_gemm_fp64(M, N, K, ta, tb):
 
 
 N_FP64_TILES = 17  # configurations of the generic MFMA engine
-N_BF16_TILES = 5   # configurations of the BF16 hi/lo kernel
+N_BF16_TILES = 6   # configurations of the BF16 hi/lo kernel
 
 
 @pytest.mark.parametrize("elementwise", [False, True])
