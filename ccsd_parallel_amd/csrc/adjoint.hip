@@ -407,13 +407,22 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   auto mm = [&](const GemmDesc<double>& g, const float* A32, const float* B32) -> int {
     return lowp ? f32(f32_desc(g, A32, B32)) : run_gemm(c, g, W);
   };
+  // The amplitude cotangents (t1b, t2b and the layouts that fold into them:
+  // taub, tauub, T2ub, Tvb, Tringb) are needed by the Lambda iterations only.
+  // With th (the gradient pass) nothing reads them -- no intermediate or
+  // integral cotangent is formed from them -- so every product that only
+  // feeds them is skipped there (the ring, ladder, W_mbej and W_mbij ones
+  // among them: four O(N^6) products).
+  const bool need_t = th == nullptr;
   if (lowp) TRY(lowprec_adj_lambda(c, lam));
-  zero(A->tb, c->nvec);
-  zero(A->taub, po * pv);
-  zero(A->T2ub, O * O * pv);
-  zero(A->tauub, O * O * pv);
-  zero(A->Tvb, po * VV);
-  zero(A->Tringb, ov * ov);
+  if (need_t) {
+    zero(A->tb, c->nvec);
+    zero(A->taub, po * pv);
+    zero(A->T2ub, O * O * pv);
+    zero(A->tauub, O * O * pv);
+    zero(A->Tvb, po * VV);
+    zero(A->Tringb, ov * ov);
+  }
   zero(A->Fmeb, ov);
   zero(A->Fmib, O * O);
   zero(A->Faeb, VV);
@@ -431,15 +440,15 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   }
 
   // -- T1 right-hand side
-  LAUNCH(r1init_vjp_kernel, ov, t1b, lam1, c->f, d);  // t1 (1-delta) f_ae
+  if (need_t) LAUNCH(r1init_vjp_kernel, ov, t1b, lam1, c->f, d);  // t1 (1-delta) f_ae
   {  // -1/2 tau_im^ef <ma||ef>, summed over m
     GemmDesc<double> g = G64(lam1, V, 1, c->ovvv, pv, 1, A->tauub, O * pv, 1, O, pv, V, -1.0, 1.0);
     g.nb1 = (int)O; g.sB1 = V * pv; g.sC1 = pv;
-    if (lowp) {
+    if (need_t && lowp) {
       GemmDesc<float> f = f32_desc(g, L->lam1_f, L->ovvv32);
       f.sBk = L->ld_pv; f.sB1 = V * L->ld_pv;
       TRY(f32(f));
-    } else {
+    } else if (need_t) {
       TRY(run_gemm(c, g, W));
     }
     if (th) {
@@ -451,18 +460,20 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   // + W3 . Tv_flat
   TRY(mm(G64(lam1, V, 1, c->Tv, 1, V, A->W3b, po * V, 1, O, po * V, V, 1.0, 1.0), L ? L->lam1_f : nullptr,
          L ? L->Tv_f : nullptr));
-  TRY(mm(G64(c->W3, 1, po * V, lam1, V, 1, A->Tvb, V, 1, po * V, V, O, 1.0, 1.0), L ? L->W3_f : nullptr,
-         L ? L->lam1_f : nullptr));
+  if (need_t)
+    TRY(mm(G64(c->W3, 1, po * V, lam1, V, 1, A->Tvb, V, 1, po * V, V, O, 1.0, 1.0), L ? L->W3_f : nullptr,
+           L ? L->lam1_f : nullptr));
   // + Tring . Fme
-  LAUNCH(outer_add_kernel, ov * ov, A->Tringb, lam1, c->Fme, ov, ov, 1.0);
+  if (need_t) LAUNCH(outer_add_kernel, ov * ov, A->Tringb, lam1, c->Fme, ov, ov, 1.0);
   TRY(run_gemm(c, G64(c->Tring, ov, 1, lam1, 1, 0, A->Fmeb, 1, 0, ov, 1, ov, 1.0, 1.0), W));
   // + G . t1   (G symmetric)
-  TRY(mm(G64(c->G, ov, 1, lam1, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), L ? L->G_f : nullptr,
-         L ? L->lam1_f : nullptr));
+  if (need_t)
+    TRY(mm(G64(c->G, ov, 1, lam1, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), L ? L->G_f : nullptr,
+           L ? L->lam1_f : nullptr));
   if (th) LAUNCH(outer_add_kernel, ov * ov, th->G, lam1, t1, ov, ov, 1.0);
   // - Fmi^T . t1
   TRY(run_gemm(c, G64(t1, V, 1, lam1, 1, V, A->Fmib, O, 1, O, O, V, -1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->Fmi, O, 1, lam1, V, 1, t1b, V, 1, O, V, O, -1.0, 1.0), W));
+  if (need_t) TRY(run_gemm(c, G64(c->Fmi, O, 1, lam1, V, 1, t1b, V, 1, O, V, O, -1.0, 1.0), W));
   if (th) {
     LAUNCH(scale_add_kernel, ov, th->fov, lam1, 1.0, ov);
     // (1-delta) f_ae: fvv_bar[a,e] += sum_i lam1[i,a] t1[i,e]  (diagonal dropped on output)
@@ -476,12 +487,12 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
     g.nb1 = (int)O; g.sA1 = O * pv; g.sB1 = O * pv; g.reduce_batch = true;
     TRY(mm(g, L ? L->T2u_f : nullptr, Zf));
   }
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(c->Fmi, O, 1, A->Zb, pv, 1, A->T2ub, pv, 1, O, pv, O, -1.0, 1.0);
     g.nb1 = (int)O; g.sB1 = O * pv; g.sC1 = O * pv;
     TRY(mm(g, L ? L->Fmi_f : nullptr, Zf));
   }
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(A->Zb, O * pv, 1, c->ovvv, 1, pv, t1b, V, 1, O, V, pv, -1.0, 1.0);
     g.nb1 = (int)O; g.sA1 = pv; g.sB1 = V * pv; g.reduce_batch = true;
     if (lowp) {
@@ -500,9 +511,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- Y[ij][a][b] = Tv[ij][a][e] Fae[b][e] - t1[m,a] W2[ij][(m,b)]
   const float* Yf = L ? L->Yb_f : nullptr;
-  TRY(mm(G64(A->Yb, V, 1, c->Fae, V, 1, A->Tvb, V, 1, po * V, V, V, 1.0, 1.0), Yf, L ? L->Fae_f : nullptr));
+  if (need_t) TRY(mm(G64(A->Yb, V, 1, c->Fae, V, 1, A->Tvb, V, 1, po * V, V, V, 1.0, 1.0), Yf, L ? L->Fae_f : nullptr));
   TRY(mm(G64(A->Yb, 1, V, c->Tv, V, 1, A->Faeb, V, 1, V, V, po * V, 1.0, 1.0), Yf, L ? L->Tv_f : nullptr));
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(c->W2, V, 1, A->Yb, 1, V, t1b, V, 1, O, V, V, -1.0, 1.0);
     g.nb1 = (int)po; g.sA1 = ov; g.sB1 = VV; g.reduce_batch = true;
     TRY(mm(g, L ? L->W2_f : nullptr, Yf));
@@ -517,13 +528,13 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   if (lowp) {
     TRY(lowprec_adj_ring(c, A->Tringb, A->WTb, lam2));
   } else {
-    TRY(run_gemm(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
+    if (need_t) TRY(run_gemm(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
     TRY(run_gemm(c, G64(A->Xb, 1, ov, c->Tring, ov, 1, A->WTb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
   }
 
   // -- Rp = <ab||ij> + tau . vvvv + W1^T . tau     (cotangent lam2)
   if (lowp) TRY(lowprec_adj_ladder(c, A->taub, lam2));
-  else TRY(run_gemm(c, G64(lam2, pv, 1, c->vvvv, pv, 1, A->taub, pv, 1, po, pv, pv, 1.0, 1.0), W));
+  else if (need_t) TRY(run_gemm(c, G64(lam2, pv, 1, c->vvvv, pv, 1, A->taub, pv, 1, po, pv, pv, 1.0, 1.0), W));
   {
     GemmDesc<double> g = G64(c->tau, pv, 1, lam2, 1, pv, A->W1b, po, 1, po, po, pv, 1.0, 1.0);
     if (lowp) {
@@ -534,7 +545,7 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
       TRY(run_gemm(c, g, W));
     }
   }
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(c->W1, po, 1, lam2, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0);
     if (lowp) {
       GemmDesc<float> f = f32_desc(g, L->W1_f, L->lam2_f);
@@ -548,10 +559,11 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- F_ae = (1-d) f + Fae_t1 - Fme^T t1 - Tv_flat^T oovv_vu_flat
   TRY(run_gemm(c, G64(t1, V, 1, A->Faeb, V, 1, A->Fmeb, V, 1, O, V, V, -1.0, 1.0), W));
-  TRY(run_gemm(c, G64(c->Fme, V, 1, A->Faeb, 1, V, t1b, V, 1, O, V, V, -1.0, 1.0), W));
+  if (need_t) TRY(run_gemm(c, G64(c->Fme, V, 1, A->Faeb, 1, V, t1b, V, 1, O, V, V, -1.0, 1.0), W));
   if (lowp) k_to_f32(L->Faeb_f, A->Faeb, VV, s);  // F_ae's cotangent is complete here
-  TRY(mm(G64(c->oovv_vu, V, 1, A->Faeb, 1, V, A->Tvb, V, 1, po * V, V, V, -1.0, 1.0), L ? L->oovv_vu_f : nullptr,
-         L ? L->Faeb_f : nullptr));
+  if (need_t)
+    TRY(mm(G64(c->oovv_vu, V, 1, A->Faeb, 1, V, A->Tvb, V, 1, po * V, V, V, -1.0, 1.0), L ? L->oovv_vu_f : nullptr,
+           L ? L->Faeb_f : nullptr));
   if (th) {
     TRY(run_gemm(c, G64(c->Tv, V, 1, A->Faeb, V, 1, th->oovv_vu, V, 1, po * V, V, V, -1.0, 1.0), W));
     LAUNCH(scale_add_kernel, VV, th->fvv, A->Faeb, 1.0, VV);
@@ -559,11 +571,13 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- F_mi = (1-d) f + t1 <mn||ie> + oovv_ou . T2u^T + Fme . t1^T
   TRY(run_gemm(c, G64(A->Fmib, O, 1, t1, V, 1, A->Fmeb, V, 1, O, V, O, 1.0, 1.0), W));
-  TRY(run_gemm(c, G64(A->Fmib, 1, O, c->Fme, V, 1, t1b, V, 1, O, V, O, 1.0, 1.0), W));
+  if (need_t) TRY(run_gemm(c, G64(A->Fmib, 1, O, c->Fme, V, 1, t1b, V, 1, O, V, O, 1.0, 1.0), W));
   if (lowp) k_to_f32(L->Fmib_f, A->Fmib, O * O, s);  // complete here
-  TRY(mm(G64(A->Fmib, 1, O, c->oovv_ou, O * pv, 1, A->T2ub, O * pv, 1, O, O * pv, O, 1.0, 1.0),
-         L ? L->Fmib_f : nullptr, L ? L->oovv_ou_f : nullptr));
-  hipLaunchKernelGGL(fmi_t1_vjp_kernel, dim3(ov), dim3(kThreads), 0, s, t1b, A->Fmib, c->ooov, d);
+  if (need_t) {
+    TRY(mm(G64(A->Fmib, 1, O, c->oovv_ou, O * pv, 1, A->T2ub, O * pv, 1, O, O * pv, O, 1.0, 1.0),
+           L ? L->Fmib_f : nullptr, L ? L->oovv_ou_f : nullptr));
+    hipLaunchKernelGGL(fmi_t1_vjp_kernel, dim3(ov), dim3(kThreads), 0, s, t1b, A->Fmib, c->ooov, d);
+  }
   if (th) {
     TRY(run_gemm(c, G64(A->Fmib, O, 1, c->T2u, O * pv, 1, th->oovv_ou, O * pv, 1, O, O * pv, O, 1.0, 1.0), W));
     LAUNCH(fmi_ooov_kernel, po * ov, th->ooov, A->Fmib, t1, d);
@@ -572,10 +586,10 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- W_mbej (WT): (c) + 1/2 Tring . Doovv
   if (lowp) TRY(lowprec_adj_wmbej(c, A->Tringb, A->WTb));
-  else TRY(run_gemm(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), W));
+  else if (need_t) TRY(run_gemm(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), W));
   if (th) TRY(run_gemm(c, G64(c->Tring, ov, 1, A->WTb, ov, 1, th->Doovv, ov, 1, ov, ov, ov, 0.5, 1.0), W));
   // (b) + t1[n,b] W3u[j][m][n][e], batched over (j, m)
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(c->W3u, V, 1, A->WTb, 1, ov, t1b, V, 1, O, V, V, 1.0, 1.0);
     g.nb1 = (int)O; g.nb2 = (int)O; g.sA1 = O * O * V; g.sA2 = O * V; g.sB1 = V * ov; g.sB2 = V;
     g.reduce_batch = true;
@@ -594,7 +608,7 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
     TRY(f32(f32_desc(G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), L->Sb_f, L->ovvv_u_f)));
   } else {
     LAUNCH(sbar_kernel<double>, O * VV * O, Sb, A->WTb, A->Faeb, d);
-    TRY(run_gemm(c, G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), W));
+    if (need_t) TRY(run_gemm(c, G64(Sb, 1, O, c->ovvv_u, V, 1, t1b, V, 1, O, V, O * VV, 1.0, 1.0), W));
   }
   if (th) TRY(run_gemm(c, G64(Sb, O, 1, t1, V, 1, th->ovvv_u, V, 1, O * VV, V, O, 1.0, 1.0), W));
 
@@ -605,7 +619,7 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   else
     hipLaunchKernelGGL(unpack_pair_rows_kernel<double>, dim3(grid_for(O * O * ov)), dim3(kThreads), 0, s, A->Ub,
                        A->W2b, (int)O, ov);
-  {
+  if (need_t) {
     GemmDesc<double> g = G64(A->Ub, O * ov, 1, c->G, 1, ov, t1b, V, 1, O, V, V, 1.0, 1.0);
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = m, b2 = j
     g.sA1 = V; g.sA2 = ov; g.sB1 = V * ov; g.sB2 = V; g.reduce_batch = true;
@@ -618,7 +632,7 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
     TRY(run_gemm(c, g, W));
   }
   if (lowp) TRY(lowprec_adj_w2(c, A->taub, A->W2b));
-  else TRY(run_gemm(c, G64(A->W2b, ov, 1, c->ovvv, pv, 1, A->taub, pv, 1, po, pv, ov, 1.0, 1.0), W));
+  else if (need_t) TRY(run_gemm(c, G64(A->W2b, ov, 1, c->ovvv, pv, 1, A->taub, pv, 1, po, pv, ov, 1.0, 1.0), W));
   if (th) {
     TRY(run_gemm(c, G64(A->W2b, 1, ov, c->tau, pv, 1, th->ovvv, pv, 1, ov, pv, po, 1.0, 1.0), W));
     LAUNCH(scale_add_kernel, po * ov, th->ooov, A->W2b, 1.0, po * ov);
@@ -628,8 +642,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   hipLaunchKernelGGL(pack_pair_rows_adj_kernel, dim3(grid_for(O * po * V)), dim3(kThreads), 0, s, A->W3b, A->W3ub,
                      (int)O, V, O);
   if (lowp) k_to_f32(L->W3b_f, A->W3b, O * po * V, s);  // complete here
-  TRY(mm(G64(A->W3b, po * V, 1, c->oovv_vu, V, 1, t1b, V, 1, O, V, po * V, -1.0, 1.0), L ? L->W3b_f : nullptr,
-         L ? L->oovv_vu_f : nullptr));
+  if (need_t)
+    TRY(mm(G64(A->W3b, po * V, 1, c->oovv_vu, V, 1, t1b, V, 1, O, V, po * V, -1.0, 1.0), L ? L->W3b_f : nullptr,
+           L ? L->oovv_vu_f : nullptr));
   if (th) {
     GemmDesc<double> g = G64(t1, 1, V, A->W3b, po * V, 1, th->oovv_vu, V, 1, V, V, O, 1.0, 1.0);
     g.nb1 = (int)po; g.sB1 = V; g.sC1 = VV;
@@ -640,10 +655,11 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   // -- W_mnij: W1 = oooo + oovv . tau^T + P(ij) Q,  Q[(mn,i)][j] = ooov[(mn,i)][e] t1[j][e]
   hipLaunchKernelGGL(unpack_pair_cols_kernel, dim3(grid_for(po * O * O)), dim3(kThreads), 0, s, A->Qb, A->W1b, po,
                      (int)O);
-  TRY(run_gemm(c, G64(A->Qb, 1, O, c->ooov, V, 1, t1b, V, 1, O, V, po * O, 1.0, 1.0), W));
+  if (need_t) TRY(run_gemm(c, G64(A->Qb, 1, O, c->ooov, V, 1, t1b, V, 1, O, V, po * O, 1.0, 1.0), W));
   if (lowp) k_to_f32(L->W1b_f, A->W1b, po * po, s);  // complete here
-  TRY(mm(G64(A->W1b, 1, po, c->oovv, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), L ? L->W1b_f : nullptr,
-         L ? L->oovv_f : nullptr));
+  if (need_t)
+    TRY(mm(G64(A->W1b, 1, po, c->oovv, pv, 1, A->taub, pv, 1, po, pv, po, 1.0, 1.0), L ? L->W1b_f : nullptr,
+           L ? L->oovv_f : nullptr));
   if (th) {
     TRY(run_gemm(c, G64(A->Qb, O, 1, t1, V, 1, th->ooov, V, 1, po * O, V, O, 1.0, 1.0), W));
     TRY(run_gemm(c, G64(A->W1b, po, 1, c->tau, pv, 1, th->oovv, pv, 1, po, pv, po, 1.0, 1.0), W));
@@ -651,13 +667,17 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   }
 
   // -- F_me = f_me + Doovv . t1
-  TRY(run_gemm(c, G64(c->Doovv, ov, 1, A->Fmeb, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), W));
+  if (need_t) TRY(run_gemm(c, G64(c->Doovv, ov, 1, A->Fmeb, 1, 0, t1b, 1, 0, ov, 1, ov, 1.0, 1.0), W));
   if (th) {
     LAUNCH(outer_add_kernel, ov * ov, th->Doovv, A->Fmeb, t1, ov, ov, 1.0);
     LAUNCH(scale_add_kernel, ov, th->fov, A->Fmeb, 1.0, ov);
   }
 
   // -- amplitude layouts: t2_bar = pack^T(Tring_bar) + pack^T(Tv_bar) + pack^T(T2u_bar); tau_bar += pack^T(tauu_bar)
+  if (!need_t) {
+    HIPTRY(hipGetLastError());
+    return CCSD_OK;
+  }
   k_assemble_t2(t2b, nullptr, t2b, A->Tringb, A->Tvb, A->T2ub, c->f, nullptr, 0, d, s);
   hipLaunchKernelGGL(pack_pair_rows_adj_kernel, dim3(grid_for(po * pv)), dim3(kThreads), 0, s, A->taub, A->tauub,
                      (int)O, pv, 1L);
