@@ -16,9 +16,10 @@
 //  * With opt.precision = BF16X3 the Lambda iterations run their five O(N^6)
 //    reverse products (ring x2, W_mbej, pp ladder, W_mbij) as BF16x3 GEMMs
 //    until max|dLambda| < lambda_switch_factor * lambda_tol (or until it stalls:
-//    no new smallest max|dLambda| in kLowpStall iterations), then at least
-//    final_fp64_sweeps FP64 iterations -- the forward's schedule, with the
-//    DIIS subspace kept across the switch.  The gradient pass is always FP64.
+//    no new smallest max|dLambda| in kLowpStall iterations), then refine: FP64
+//    passes, each followed (until one meets max|dLambda| < lambda_tol) by a
+//    BF16x3 solve of the linear correction equation.  The gradient pass is
+//    always FP64.
 // The reference has no adjoint; its forward algebra is restated in engine.hip.
 #include <algorithm>
 #include <chrono>
@@ -34,6 +35,9 @@ namespace ccsd {
 struct AdjState {
   double *e = nullptr;                      // dE/dT  [t1 | t2 packed]
   double *lam = nullptr, *lnew = nullptr, *lerr = nullptr, *tb = nullptr;  // nvec each
+  // iterative refinement of a mixed run: the correction and its constant part
+  // (nvec each, allocated on first use)
+  double *dcorr = nullptr, *rcorr = nullptr;
   double *taub = nullptr, *T2ub = nullptr, *tauub = nullptr, *Tvb = nullptr, *Tringb = nullptr;
   double *Fmeb = nullptr, *Fmib = nullptr, *Faeb = nullptr, *W1b = nullptr, *Qb = nullptr;
   double *W3b = nullptr, *W3ub = nullptr, *W2b = nullptr, *Ub = nullptr, *WTb = nullptr;
@@ -213,25 +217,39 @@ __global__ void fmi_t1_vjp_kernel(double* t1b, const double* Fmib, const double*
 }
 
 // lnew = (e + jt) / D ; err = lnew - lam    (packed T2 and T1)
+// Orbital-energy denominator of element idx of [t1 | packed t2].
+__device__ __forceinline__ double lambda_denominator(long idx, const double* f, const Dims& d) {
+  const long N = d.O + d.V;
+  if (idx < d.ov) {
+    const int i = (int)(idx / d.V), a = (int)(idx % d.V);
+    return f[(long)i * N + i] - f[(long)(d.O + a) * N + d.O + a];
+  }
+  const long q = idx - d.ov;
+  int i, j, a, b;
+  pair_decode(q / d.pv, i, j);
+  pair_decode(q % d.pv, a, b);
+  return f[(long)i * N + i] + f[(long)j * N + j] - f[(long)(d.O + a) * N + d.O + a] - f[(long)(d.O + b) * N + d.O + b];
+}
+
 __global__ void lambda_update_kernel(double* lnew, double* err, const double* e, const double* jt, const double* lam,
                                      const double* f, Dims d) {
-  const long N = d.O + d.V;
   const long n = d.ov + d.po * d.pv;
   GRID_LOOP(idx, n) {
-    double D;
-    if (idx < d.ov) {
-      const int i = (int)(idx / d.V), a = (int)(idx % d.V);
-      D = f[(long)i * N + i] - f[(long)(d.O + a) * N + d.O + a];
-    } else {
-      const long q = idx - d.ov;
-      int i, j, a, b;
-      pair_decode(q / d.pv, i, j);
-      pair_decode(q % d.pv, a, b);
-      D = f[(long)i * N + i] + f[(long)j * N + j] - f[(long)(d.O + a) * N + d.O + a] - f[(long)(d.O + b) * N + d.O + b];
-    }
-    const double v = (e[idx] + jt[idx]) / D;
+    const double v = (e[idx] + jt[idx]) / lambda_denominator(idx, f, d);
     lnew[idx] = v;
     err[idx] = v - lam[idx];
+  }
+}
+
+// Correction equation of the refinement, dcorr = r0 + T(dcorr) / D:
+// dnew = r0 + jt / D ; err = dnew - dcur   (jt = T(dcur) from the reverse pass)
+__global__ void correction_update_kernel(double* dnew, double* err, const double* r0, const double* jt,
+                                         const double* dcur, const double* f, Dims d) {
+  const long n = d.ov + d.po * d.pv;
+  GRID_LOOP(idx, n) {
+    const double v = r0[idx] + jt[idx] / lambda_denominator(idx, f, d);
+    dnew[idx] = v;
+    err[idx] = v - dcur[idx];
   }
 }
 
@@ -718,24 +736,103 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   HIPTRY(hipMemcpyAsync(A->e, c->fov, ov * 8, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(tau_vjp_t1_kernel, dim3(ov), dim3(kThreads), 0, s, A->e, c->oovv, t1, d);
 
-  // Lambda iterations (Jacobi + DIIS on the forward's DIIS buffers); BF16x3
-  // reverse products until the switch, then FP64 sweeps
+  // Lambda iterations (Jacobi + DIIS on the forward's DIIS buffers).  FP64
+  // runs: FP64 sweeps until max|dLambda| < tol.  BF16x3 runs: BF16x3 reverse
+  // products until the switch, then iterative refinement -- the equations are
+  // linear, Lambda = (e + T(Lambda)) / D, so at the current Lambda one FP64
+  // pass gives the exact update difference r0 = (e + T(Lambda)) / D - Lambda,
+  // and the correction dLambda = r0 + T(dLambda) / D is solved with BF16x3
+  // reverse passes (their rounding scales with |dLambda|, orders of magnitude
+  // below the tolerance); Lambda += dLambda, and the next FP64 pass either
+  // meets max|dLambda| < tol (converged: the FP64 equations hold to tol, as in
+  // an FP64 run) or starts another correction.  At O=40/V=300 this takes two
+  // FP64 passes where FP64 sweeps with the BF16x3 phase's DIIS subspace took four.
   LAUNCH(zero_kernel, c->nvec, A->lam, c->nvec);
   const int m = c->diis_m;
-  std::vector<double> B((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
-  int count = 0, next = 0, it = 0, lowp_its = 0, fp64_after = c->opt.final_fp64_sweeps;
+  struct Diis {
+    int count = 0, next = 0;
+    std::vector<double> B;
+  };
+  Diis dl;
+  dl.B.assign((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
+  // Pulay step on the forward's DIIS buffers: stores (vnew, verr) and writes
+  // the extrapolation into *out, or swaps vnew into *out without one
+  auto diis_step = [&](Diis& st, double*& vnew, const double* verr, double*& out) -> int {
+    bool extrap = false;
+    if (m >= 2) {
+      const int slot = st.next;
+      HIPTRY(hipMemcpyAsync(c->dvec + (long)slot * c->nvec, vnew, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+      HIPTRY(hipMemcpyAsync(c->derr + (long)slot * c->nvec, verr, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+      st.count = std::min(st.count + 1, m);
+      st.next = (slot + 1) % m;
+      const int nbd = reduce_blocks_for(c->nvec);
+      k_multidot_partials(c->partial, c->derr, c->nvec, st.count, verr, c->nvec, nbd, s);
+      k_sum_partials(c->scal, c->partial, nbd, st.count, s);
+      std::vector<double> row(st.count);
+      HIPTRY(hipMemcpyAsync(row.data(), c->scal, st.count * 8, hipMemcpyDeviceToHost, s));
+      HIPTRY(hipStreamSynchronize(s));
+      for (int k = 0; k < st.count; ++k) { st.B[(size_t)slot * m + k] = row[k]; st.B[(size_t)k * m + slot] = row[k]; }
+      if (st.count >= 2) {
+        std::vector<double> Bc((size_t)st.count * st.count), coef;
+        for (int i = 0; i < st.count; ++i)
+          for (int j = 0; j < st.count; ++j) Bc[(size_t)i * st.count + j] = st.B[(size_t)i * m + j];
+        if (diis_solve_bordered(Bc, st.count, coef)) {
+          HIPTRY(hipMemcpyAsync(c->dcoef, coef.data(), st.count * 8, hipMemcpyHostToDevice, s));
+          k_lincomb(out, c->dvec, c->nvec, c->dcoef, st.count, c->nvec, s);
+          extrap = true;
+        }
+      }
+    }
+    if (!extrap) std::swap(out, vnew);
+    return CCSD_OK;
+  };
+  int it = 0, lowp_its = 0, corr_its = 0, fp64_after = c->opt.final_fp64_sweeps;
   double delta = 1e300;
   // Lambda tolerance: the option, else 1e-2 x the forward's (clamped): the
   // gradient's error is then set by where the amplitudes stopped
   const double tol = c->opt.lambda_tol > 0.0 ? c->opt.lambda_tol
                                               : std::min(1e-8, std::max(1e-11, 1e-2 * c->last_tol));
-  bool conv = false, lowp = c->opt.precision == CCSD_PREC_BF16X3;
+  bool conv = false, lowp = c->opt.precision == CCSD_PREC_BF16X3, refine = false;
   double best_low = HUGE_VAL;  // smallest max|dLambda| of the BF16x3 phase
   int since_best = 0;
   if (lowp) {
     TRY(lowprec_adj_setup(c));
     fp64_after = 0;
   }
+  // One correction solve (BF16x3, fresh DIIS subspace) from r0 = lerr, then
+  // lam += dcorr.  Stops once max|d(dcorr)| < kCorrTol * tol, or when it stalls.
+  constexpr double kCorrTol = 0.05;
+  constexpr int kCorrMax = 20;
+  auto correct = [&]() -> int {
+    if (!A->dcorr) {
+      TRY(adj_alloc(c, A, &A->dcorr, c->nvec));
+      TRY(adj_alloc(c, A, &A->rcorr, c->nvec));
+    }
+    HIPTRY(hipMemcpyAsync(A->rcorr, A->lerr, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipMemcpyAsync(A->dcorr, A->lerr, c->nvec * 8, hipMemcpyDeviceToDevice, s));
+    Diis dc;
+    dc.B.assign((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
+    double best = HUGE_VAL, dd = HUGE_VAL;
+    int since = 0;
+    for (int k = 0; k < kCorrMax; ++k) {
+      TRY(reverse_pass(c, A->dcorr, nullptr, true));
+      ++lowp_its;
+      ++corr_its;
+      LAUNCH(correction_update_kernel, c->nvec, A->lnew, A->lerr, A->rcorr, A->tb, A->dcorr, c->f, d);
+      TRY(maxabs(c, A->lerr, c->nvec, &dd));
+      if (!std::isfinite(dd)) return CCSD_ERR_NONFINITE;
+      TRY(diis_step(dc, A->lnew, A->lerr, A->dcorr));
+      if (dd < kCorrTol * tol) break;
+      if (dd < best) {
+        best = dd;
+        since = 0;
+      } else if (++since >= kLowpStall) {
+        break;
+      }
+    }
+    LAUNCH(scale_add_kernel, c->nvec, A->lam, A->dcorr, 1.0, c->nvec);
+    return CCSD_OK;
+  };
   for (it = 1; it <= 500; ++it) {
     TRY(reverse_pass(c, A->lam, nullptr, lowp));
     if (lowp) ++lowp_its;
@@ -743,36 +840,17 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
     LAUNCH(lambda_update_kernel, c->nvec, A->lnew, A->lerr, A->e, A->tb, A->lam, c->f, d);
     TRY(maxabs(c, A->lerr, c->nvec, &delta));
     if (!std::isfinite(delta)) return CCSD_ERR_NONFINITE;
-    bool extrap = false;
-    if (m >= 2) {
-      const int slot = next;
-      HIPTRY(hipMemcpyAsync(c->dvec + (long)slot * c->nvec, A->lnew, c->nvec * 8, hipMemcpyDeviceToDevice, s));
-      HIPTRY(hipMemcpyAsync(c->derr + (long)slot * c->nvec, A->lerr, c->nvec * 8, hipMemcpyDeviceToDevice, s));
-      count = std::min(count + 1, m);
-      next = (slot + 1) % m;
-      const int nbd = reduce_blocks_for(c->nvec);
-      k_multidot_partials(c->partial, c->derr, c->nvec, count, A->lerr, c->nvec, nbd, s);
-      k_sum_partials(c->scal, c->partial, nbd, count, s);
-      std::vector<double> row(count);
-      HIPTRY(hipMemcpyAsync(row.data(), c->scal, count * 8, hipMemcpyDeviceToHost, s));
-      HIPTRY(hipStreamSynchronize(s));
-      for (int k = 0; k < count; ++k) { B[(size_t)slot * m + k] = row[k]; B[(size_t)k * m + slot] = row[k]; }
-      if (count >= 2) {
-        std::vector<double> Bc((size_t)count * count), coef;
-        for (int i = 0; i < count; ++i)
-          for (int j = 0; j < count; ++j) Bc[(size_t)i * count + j] = B[(size_t)i * m + j];
-        if (diis_solve_bordered(Bc, count, coef)) {
-          HIPTRY(hipMemcpyAsync(c->dcoef, coef.data(), count * 8, hipMemcpyHostToDevice, s));
-          k_lincomb(A->lam, c->dvec, c->nvec, c->dcoef, count, c->nvec, s);
-          extrap = true;
-        }
+    if (refine) {
+      if (delta < tol) {
+        std::swap(A->lam, A->lnew);
+        conv = true;
+        break;
       }
+      TRY(correct());
+      continue;
     }
-    if (!extrap) std::swap(A->lam, A->lnew);
+    TRY(diis_step(dl, A->lnew, A->lerr, A->lam));
     if (lowp) {
-      // refine in FP64 from here on; the DIIS subspace is kept (at O=40/V=300:
-      // 15 + 5 iterations against 15 + 7 with a reset), since the
-      // max|dLambda| < tol test on FP64 updates pins the FP64 fixed point
       const double f = c->opt.lambda_switch_factor > 0 ? c->opt.lambda_switch_factor : c->opt.switch_factor;
       if (delta < best_low) {
         best_low = delta;
@@ -780,8 +858,12 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
       } else {
         ++since_best;
       }
-      // also switch once the BF16x3 phase stalls at its rounding floor
-      if (delta < std::max(f * tol, tol) || since_best >= kLowpStall) lowp = false;
+      // switch once max|dLambda| is near the tolerance, or once the BF16x3
+      // phase stalls at its rounding floor
+      if (delta < std::max(f * tol, tol) || since_best >= kLowpStall) {
+        lowp = false;
+        refine = true;
+      }
       continue;
     }
     if (delta < tol && fp64_after >= c->opt.final_fp64_sweeps) { conv = true; break; }
@@ -941,7 +1023,7 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   for (long p = 0; p < N; ++p)
     for (long q = 0; q < N; ++q) g.fock[p * N + q] = 0.5 * (G[p * N + q] + G[q * N + p]);
   if (info) {
-    info->lambda_iterations = std::min(it, 500);
+    info->lambda_iterations = std::min(it, 500) + corr_its;
     info->lambda_lowp_iterations = lowp_its;
     info->converged = conv ? 1 : 0;
     info->lambda_residual = delta;

@@ -148,8 +148,10 @@ def test_backward_finite_differences_o20_v180():
 
 
 def test_backward_mixed_precision_matches_fp64():
-    """BF16x3 forward and Lambda iterations (each ending with FP64 sweeps) give
-    the FP64 gradient: relative error of the whole gradient below 1e-6."""
+    """BF16x3 forward (ending with FP64 sweeps) and BF16x3 Lambda iterations
+    (ending with FP64-verified refinement: the last pass is an FP64 one that
+    meets the tolerance) give the FP64 gradient: relative error of the whole
+    gradient below 1e-6."""
     capi = _capi()
     O, V, seed = 10, 40, 35
     sc = 1.0 / np.sqrt(O * synth.p2(V))
@@ -161,7 +163,7 @@ def test_backward_mixed_precision_matches_fp64():
         out[prec] = (r, g, info)
     (r64, g64, i64), (rmx, gmx, imx) = out["fp64"], out["bf16x3"]
     assert i64["lambda_lowp_iterations"] == 0 and imx["lambda_lowp_iterations"] > 0
-    assert imx["converged"] == 1 and imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 3
+    assert imx["converged"] == 1 and imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 1
     num = np.sqrt(sum(float(np.sum((gmx[k] - g64[k]) ** 2)) for k in KEYS))
     den = np.sqrt(sum(float(np.sum(g64[k] ** 2)) for k in KEYS))
     assert num / den <= 1e-6, num / den
@@ -170,9 +172,9 @@ def test_backward_mixed_precision_matches_fp64():
 
 def test_backward_mixed_switches_at_a_stall():
     """lambda_switch_factor 1e-6 puts the Lambda switch (1e-10) below the
-    BF16x3 rounding floor: the Lambda iteration switches to FP64 once
-    max|dLambda| has set no new minimum in 4 iterations, and the gradient
-    still matches the FP64 one."""
+    BF16x3 rounding floor: the Lambda iteration switches to the FP64-verified
+    refinement once max|dLambda| has set no new minimum in 4 iterations, and
+    the gradient still matches the FP64 one."""
     capi = _capi()
     O, V, seed = 8, 24, 37
     sc = 1.0 / np.sqrt(O * synth.p2(V))
@@ -184,7 +186,7 @@ def test_backward_mixed_switches_at_a_stall():
         out[prec] = (g, info)
     (g64, _), (gmx, imx) = out["fp64"], out["bf16x3"]
     assert imx["converged"] == 1 and imx["lambda_lowp_iterations"] > 0
-    assert imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 3
+    assert imx["lambda_iterations"] - imx["lambda_lowp_iterations"] >= 1
     num = np.sqrt(sum(float(np.sum((gmx[k] - g64[k]) ** 2)) for k in KEYS))
     den = np.sqrt(sum(float(np.sum(g64[k] ** 2)) for k in KEYS))
     assert num / den <= 1e-6, num / den

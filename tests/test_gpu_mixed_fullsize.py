@@ -7,7 +7,9 @@
   the same tolerance (max|dT2| and max|dT1| within 2x of the FP64 run's, plus
   a 1e-8 floor).
 * Forward + adjoint: BF16x3-mixed forward (to 1e-7) and BF16x3-mixed Lambda
-  iterations give dE/df and dE/d<pq||rs> within 1e-6 relative (Frobenius norm
+  iterations (BF16x3 sweeps, then FP64 passes each followed by a BF16x3
+  correction solve until an FP64 pass meets the Lambda tolerance) give
+  dE/df and dE/d<pq||rs> within 1e-6 relative (Frobenius norm
   over every block) of the FP64 forward (to 1e-10) and FP64 adjoint.  The
   22 GB of gradients per run land in page-locked host arrays and are compared
   in row chunks.
@@ -77,7 +79,7 @@ def test_mixed_forward_adjoint_o40_v300_matches_fp64():
             assert info["converged"] == 1
             if prec == "bf16x3":
                 assert r["lowp_iterations"] > 0 and info["lambda_lowp_iterations"] > 0
-                assert info["lambda_iterations"] - info["lambda_lowp_iterations"] >= 3
+                assert info["lambda_iterations"] - info["lambda_lowp_iterations"] >= 1
         a, b = bufs["fp64"].arrays, bufs["bf16x3"].arrays
         num = den = 0.0
         worst = {}
