@@ -801,7 +801,9 @@ int adjoint_run(Ctx* c, const GradOut& g, ccsd_grad_info* info) {
   }
   // One correction solve (BF16x3, fresh DIIS subspace) from r0 = lerr, then
   // lam += dcorr.  Stops once max|d(dcorr)| < kCorrTol * tol, or when it stalls.
-  constexpr double kCorrTol = 0.05;
+  // (O=40/V=300, tol 1e-9: 0.3 takes 5 passes and the verifying FP64 pass
+  // lands at 1.1e-10; 0.1 and 0.05 take 7, at 9e-12 -- same gradient.)
+  constexpr double kCorrTol = 0.3;
   constexpr int kCorrMax = 20;
   auto correct = [&]() -> int {
     if (!A->dcorr) {
