@@ -317,46 +317,74 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4) ? 2 : 1) void bf16x3_n
   // l & 15, so its swizzle is (l & 7) ^ ((l >> 3) & 1)), k-slice l >> 4
   const int swz = (lane & 7) ^ ((lane >> 3) & 1), ks = lane >> 4;
   const int oh = ((2 * ks) ^ swz) << 3, ol = ((2 * ks + 1) ^ swz) << 3;
-  auto compute = [&](const uint16_t* a, const uint16_t* b) {
-    bf16x8_t ah[TM], al[TM];
+  struct AFrag { bf16x8_t h[TM], l[TM]; };
+  struct BFrag { bf16x8_t h, l; };
+  auto read_a = [&](const uint16_t* a, AFrag& f) {
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int r = (wm0 + i * 16 + (lane & 15)) * RP;
-      ah[i] = *(const bf16x8_t*)&a[r + oh];
-      al[i] = *(const bf16x8_t*)&a[r + ol];
-    }
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int r = (wn0 + j * 16 + (lane & 15)) * RP;
-      const bf16x8_t bh = *(const bf16x8_t*)&b[r + oh], bl = *(const bf16x8_t*)&b[r + ol];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+      f.l[i] = *(const bf16x8_t*)&a[r + ol];
+      f.h[i] = *(const bf16x8_t*)&a[r + oh];
     }
   };
-  // vmcnt(0): every load this wave issued (the current tile's) has written LDS
+  auto read_b = [&](const uint16_t* b, int j, BFrag& f) {
+    const int r = (wn0 + j * 16 + (lane & 15)) * RP;
+    f.h = *(const bf16x8_t*)&b[r + oh];
+    f.l = *(const bf16x8_t*)&b[r + ol];
+  };
+  // column block j: the three products of each 16 x 16 block as three passes
+  // over the wave's row blocks (dependent MFMAs TM instructions apart)
+  auto mma_j = [&](const AFrag& fa, const BFrag& fb, int j) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.l[i], fb.h, acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.h[i], fb.l, acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa.h[i], fb.h, acc[i][j], 0, 0, 0);
+  };
+  // vmcnt(0): every load this wave issued (the next tile's) has written LDS
   constexpr int kVmcnt0 = 0x3f70;
+  AFrag fa0, fa1;
+  BFrag fb[2];
+  // One k-tile t held in LDS (a, b), its A fragments in fa and its first B
+  // fragment in fb[P]: column block j reads block j+1's B fragment before its
+  // MFMAs.  Before the last block: wait for this wave's loads of tile t+1,
+  // barrier (every wave's loads of t+1 have landed and every wave has read its
+  // last fragment of t), issue the loads of tile t+2 into (a, b), and read
+  // tile t+1's A fragments into fn and its first B fragment while the last
+  // block's MFMAs run.  P is the parity of fb at block 0 (TN may be odd).
+  auto ktile = [&](int t, uint16_t* a, uint16_t* b, const uint16_t* na, const uint16_t* nb, const AFrag& fa,
+                  AFrag& fn, auto P) {
+    constexpr int p0 = decltype(P)::value;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      if (j + 1 < TN) {
+        read_b(b, j + 1, fb[(p0 + j + 1) & 1]);
+      } else {
+        // also after the last tile (uniform; the reads of the idle buffer are
+        // in bounds and unused): no branch around the reads, so the MFMAs
+        // below wait only for this block's B fragment
+        __builtin_amdgcn_s_waitcnt(kVmcnt0);
+        __syncthreads();
+        if (t + 2 < nk) issue(t + 2, a, b);
+        read_a(na, fn);
+        read_b(nb, 0, fb[(p0 + TN) & 1]);
+      }
+      __builtin_amdgcn_sched_barrier(0);  // the fragment reads (and loads) go out before this block's MFMAs
+      mma_j(fa, fb[(p0 + j) & 1], j);
+    }
+  };
   issue(0, sA0, sB0);
-  int kt = 0;
-  for (; kt + 2 <= nk; kt += 2) {
-    __builtin_amdgcn_s_waitcnt(kVmcnt0);
-    __syncthreads();
-    issue(kt + 1, sA1, sB1);
-    __builtin_amdgcn_sched_barrier(0);  // the next tile's loads go out before this tile's MFMAs
-    compute(sA0, sB0);
-    __builtin_amdgcn_s_waitcnt(kVmcnt0);
-    __syncthreads();
-    if (kt + 2 < nk) issue(kt + 2, sA0, sB0);
-    __builtin_amdgcn_sched_barrier(0);
-    compute(sA1, sB1);
-  }
-  if (kt < nk) {
-    __builtin_amdgcn_s_waitcnt(kVmcnt0);
-    __syncthreads();
-    compute(sA0, sB0);
+  if (nk > 1) issue(1, sA1, sB1);
+  __builtin_amdgcn_s_waitcnt(kVmcnt0);
+  __syncthreads();
+  read_a(sA0, fa0);
+  read_b(sB0, 0, fb[0]);
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, TN & 1>;
+  for (int kt = 0; kt < nk; kt += 2) {
+    ktile(kt, sA0, sB0, sA1, sB1, fa0, fa1, P0{});
+    if (kt + 1 < nk) ktile(kt + 1, sA1, sB1, sA0, sB0, fa1, fa0, P1{});
   }
   if (tail_out) {
     // whole BM x BN tile in tile coordinates; the reduction drops rows and

@@ -139,7 +139,8 @@ def test_gemm_bf16_split_tail(tile):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16x3", 5e-5)])
-@pytest.mark.parametrize("M,N,K", [(130, 70, 300), (256, 256, 1024), (33, 129, 65)])
+# K = 20 and 64: one and two k-tiles (the pipeline's prologue and last-tile paths)
+@pytest.mark.parametrize("M,N,K", [(130, 70, 300), (256, 256, 1024), (33, 129, 65), (70, 50, 20), (100, 90, 64)])
 def test_gemm_lowprec(prec, tol, M, N, K):
     capi = _ctx()
     rng = np.random.default_rng(M + N + K)
