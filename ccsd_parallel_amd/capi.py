@@ -25,6 +25,10 @@ EXPORTED = (
     "ccsd_backward", "ccsd_plan_bytes", "ccsd_last_timings", "ccsd_destroy", "ccsd_strerror", "ccsd_debug_get",
     "ccsd_selftest_gemm", "ccsd_update_integrals",
     "ccsd_host_alloc", "ccsd_host_free", "ccsd_create_rhf", "ccsd_ao_to_mo", "ccsd_energy_noalloc",
+    # closed-shell (spin-adapted) contexts for RHF inputs
+    "ccsd_rhf_create", "ccsd_rhf_plan_bytes", "ccsd_rhf_solve", "ccsd_rhf_iterate", "ccsd_rhf_reset",
+    "ccsd_rhf_residual", "ccsd_rhf_energy", "ccsd_rhf_get_amplitudes", "ccsd_rhf_set_amplitudes",
+    "ccsd_rhf_last_timings", "ccsd_rhf_last_error", "ccsd_rhf_destroy",
     # include/ccsd_rhf.h (host RHF input producer)
     "ccsd_basis_size", "ccsd_rhf", "ccsd_ao_one_electron",
 )
@@ -117,6 +121,21 @@ def load():
     lib.ccsd_update_integrals.argtypes = [vp, C.c_int, C.c_long, C.POINTER(C.c_long), dp]
     lib.ccsd_selftest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_double,
                                        dp, C.c_int]
+    lib.ccsd_rhf_create.argtypes = [C.POINTER(RhfProblem), C.POINTER(Options), C.POINTER(vp)]
+    lib.ccsd_rhf_plan_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.ccsd_rhf_plan_bytes.restype = C.c_size_t
+    lib.ccsd_rhf_solve.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Result)]
+    lib.ccsd_rhf_iterate.argtypes = [vp, C.c_int, dp]
+    lib.ccsd_rhf_reset.argtypes = [vp]
+    lib.ccsd_rhf_residual.argtypes = [vp, dp, dp, dp, dp]
+    lib.ccsd_rhf_energy.argtypes = [vp, dp, dp, dp]
+    lib.ccsd_rhf_get_amplitudes.argtypes = [vp, dp, dp]
+    lib.ccsd_rhf_set_amplitudes.argtypes = [vp, dp, dp]
+    lib.ccsd_rhf_last_timings.argtypes = [vp, dp, C.c_int]
+    lib.ccsd_rhf_last_error.argtypes = [vp]
+    lib.ccsd_rhf_last_error.restype = C.c_char_p
+    lib.ccsd_rhf_destroy.argtypes = [vp]
+    lib.ccsd_rhf_destroy.restype = None
     lib.ccsd_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
     lib.ccsd_host_free.argtypes = [vp]
     ip = C.POINTER(C.c_int)
@@ -419,6 +438,93 @@ def ao_one_electron(Z, R, basis, angstrom=False):
     _check(load().ccsd_ao_one_electron(len(Zc), Zc.ctypes.data_as(C.POINTER(C.c_int)), _ptr(Rc), basis.encode(),
                                        _ptr(S), _ptr(T), _ptr(V), n), "ccsd_ao_one_electron")
     return S, T, V
+
+
+class RhfContext:
+    """Closed-shell (spin-adapted) CCSD context for RHF inputs (ccsd_rhf_*):
+    amplitudes t1[o, v] and t2[o, o, v, v] (the alpha-beta block)."""
+
+    def __init__(self, eps, eri, n_occ, mo_coeff=None, n_frozen=0, **opts):
+        self.lib = lib = load()
+        self.h = None
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        eri = np.ascontiguousarray(eri, dtype=np.float64)
+        n = eps.shape[0]
+        if eri.shape != (n, n, n, n):
+            raise ValueError("eri must be n x n x n x n")
+        Cm = None if mo_coeff is None else np.ascontiguousarray(mo_coeff, dtype=np.float64)
+        p = RhfProblem(n, int(n_occ), int(n_frozen), _ptr(eps), _ptr(Cm) if Cm is not None else None, _ptr(eri))
+        o = default_options(**opts)
+        h = C.c_void_p()
+        _check(lib.ccsd_rhf_create(C.byref(p), C.byref(o), C.byref(h)), "ccsd_rhf_create")
+        self.h = h
+        self.o, self.v = int(n_occ) - int(n_frozen), n - int(n_occ)
+
+    def close(self):
+        if self.h:
+            self.lib.ccsd_rhf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, tol=1e-10, max_iter=1000, allow_unconverged=False):
+        r = Result()
+        rc = self.lib.ccsd_rhf_solve(self.h, tol, max_iter, C.byref(r))
+        if rc != 0 and not (allow_unconverged and rc == -5):
+            raise CCSDError(rc, "ccsd_rhf_solve")
+        return r.as_dict()
+
+    def iterate(self, n):
+        e = np.zeros(max(n, 1))
+        _check(self.lib.ccsd_rhf_iterate(self.h, n, _ptr(e)), "ccsd_rhf_iterate")
+        return e[:n]
+
+    def reset(self):
+        _check(self.lib.ccsd_rhf_reset(self.h), "ccsd_rhf_reset")
+
+    def residual(self, t1, t2):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2 = np.ascontiguousarray(t2, dtype=np.float64)
+        r1, r2 = np.zeros_like(t1), np.zeros_like(t2)
+        _check(self.lib.ccsd_rhf_residual(self.h, _ptr(t1), _ptr(t2), _ptr(r1), _ptr(r2)), "ccsd_rhf_residual")
+        return r1, r2
+
+    def energy(self, t1, t2):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2 = np.ascontiguousarray(t2, dtype=np.float64)
+        e = C.c_double()
+        _check(self.lib.ccsd_rhf_energy(self.h, _ptr(t1), _ptr(t2), C.byref(e)), "ccsd_rhf_energy")
+        return e.value
+
+    def get_amplitudes(self):
+        t1 = np.zeros((self.o, self.v))
+        t2 = np.zeros((self.o, self.o, self.v, self.v))
+        _check(self.lib.ccsd_rhf_get_amplitudes(self.h, _ptr(t1), _ptr(t2)), "ccsd_rhf_get_amplitudes")
+        return t1, t2
+
+    def set_amplitudes(self, t1, t2):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2 = np.ascontiguousarray(t2, dtype=np.float64)
+        _check(self.lib.ccsd_rhf_set_amplitudes(self.h, _ptr(t1), _ptr(t2)), "ccsd_rhf_set_amplitudes")
+
+    def timings(self):
+        ms = np.zeros(1)
+        self.lib.ccsd_rhf_last_timings(self.h, _ptr(ms), 1)
+        return {"residual": float(ms[0])}
+
+
+def rhf_plan_bytes(n_basis, n_occ, n_frozen=0):
+    return load().ccsd_rhf_plan_bytes(n_basis, n_occ, n_frozen)
 
 
 class PinnedArrays:

@@ -14,24 +14,11 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "gemm_desc.hpp"
+
 namespace ccsd {
 
-template <typename T>
-struct GemmDesc {
-  const T* A = nullptr;
-  const T* B = nullptr;
-  T* C = nullptr;
-  int M = 0, N = 0, K = 0;
-  long sAm = 0, sAk = 0, sBk = 0, sBn = 0, sCm = 0, sCn = 0;
-  int nb1 = 1, nb2 = 1;
-  long sA1 = 0, sA2 = 0, sB1 = 0, sB2 = 0, sC1 = 0, sC2 = 0;
-  T alpha = 1, beta = 0;
-  bool reduce_batch = false;
-  // When set, the result goes to this FP64 array (same offsets as C, which is
-  // then unused): C_d = alpha * (A B accumulated in T) + beta * C_d.  Lets a
-  // lower-precision product update an FP64 intermediate without a second pass.
-  double* Cd = nullptr;
-};
+
 
 // Scratch the engine may use for split-K / batch-reduce partials.
 struct GemmWorkspace {

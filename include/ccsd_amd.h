@@ -238,6 +238,38 @@ int ccsd_last_timings(ccsd_ctx* c, double* ms, int cap);
 void ccsd_destroy(ccsd_ctx* c);
 const char* ccsd_strerror(int code);
 
+/* ---- Closed-shell (spin-adapted) CCSD for RHF inputs ----------------------
+ * The spin-block mode of the reference's RHF path: CCSD_Memory_optimized::CCSD
+ * (ccsd_memory_optimized.cpp:16-567) iterates spin-orbital amplitudes of an
+ * RHF reference and skips the spin-forbidden blocks (:483-485); for a
+ * closed-shell reference all spin blocks follow from the spatial amplitudes
+ * t1[i][a] = t_{i alpha}^{a alpha} and t2[i][j][a][b] = t_{i alpha j beta}^{a
+ * alpha b beta} (o = n_occ - n_frozen, v = n_basis - n_occ; t2 stored in full,
+ * t2[i][j][a][b] = t2[j][i][b][a]), which this context iterates directly.
+ * Same inputs as ccsd_create_rhf; the correlation energy equals that of the
+ * spin-orbital context on the same problem.  FP64 only (CCSD_ERR_ARG for
+ * another precision).  Set-up needs the n^4 integrals in HBM (twice with C). */
+typedef struct ccsd_rhf_ctx ccsd_rhf_ctx;
+int ccsd_rhf_create(const ccsd_rhf_problem* p, const ccsd_options* o, ccsd_rhf_ctx** out);
+/* Device bytes ccsd_rhf_create needs at its peak (8-vector DIIS, AO input). */
+size_t ccsd_rhf_plan_bytes(int n_basis, int n_occ, int n_frozen);
+/* As ccsd_solve: from the MP2 guess (or ccsd_rhf_set_amplitudes) until |dE| < tol. */
+int ccsd_rhf_solve(ccsd_rhf_ctx* c, double tol, int max_iter, ccsd_result* out);
+/* n updates (with DIIS) from the current amplitudes; energies[n] may be NULL. */
+int ccsd_rhf_iterate(ccsd_rhf_ctx* c, int n, double* energies);
+/* Back to the MP2 guess, DIIS cleared. */
+int ccsd_rhf_reset(ccsd_rhf_ctx* c);
+/* Right-hand sides r1 [o][v], r2 [o][o][v][v] of D t = R at (t1, t2) (host arrays). */
+int ccsd_rhf_residual(ccsd_rhf_ctx* c, const double* t1, const double* t2, double* r1, double* r2);
+/* E = sum [2 (ia|jb) - (ib|ja)] (t2[i][j][a][b] + t1[i][a] t1[j][b]) at host amplitudes. */
+int ccsd_rhf_energy(ccsd_rhf_ctx* c, const double* t1, const double* t2, double* e);
+int ccsd_rhf_get_amplitudes(ccsd_rhf_ctx* c, double* t1, double* t2);
+int ccsd_rhf_set_amplitudes(ccsd_rhf_ctx* c, const double* t1, const double* t2);
+/* ms[0]: device time of the last right-hand side (ms). */
+int ccsd_rhf_last_timings(ccsd_rhf_ctx* c, double* ms, int cap);
+const char* ccsd_rhf_last_error(ccsd_rhf_ctx* c);
+void ccsd_rhf_destroy(ccsd_rhf_ctx* c);
+
 /* Test hook: copy the named FP64 intermediate of the last right-hand side
  * ("Fme", "Fmi", "Fae", "W1", "W2", "W3", "WT", "X", "Y", "Z", "Rp", "R1",
  * "tau", "Tring", "Tv", "T2u") into out; returns its element count or < 0. */

@@ -1,0 +1,129 @@
+// Host executor for the closed-shell CCSD term sequence (test infrastructure).
+//
+// Runs the contraction planner and the term sequence of
+// ccsd_parallel_amd/csrc/rplan.hpp on host arrays: every planned permute is a
+// strided loop, every planned GEMM descriptor is evaluated by plain loops over
+// its batches, rows, columns and k.  The tests compare the result with the
+// numpy closed-shell oracle, which checks the planner's strides and the term
+// sequence without a GPU.  Built by tests/test_rccsd_oracle.py with g++.
+#include <cstdio>
+#include <vector>
+
+#include "../../ccsd_parallel_amd/csrc/rplan.hpp"
+
+using namespace ccsd::rplan;
+
+namespace {
+
+void run_copy(const Copy& c) {
+  const long n = numel(c.dst);
+  long idx[kMaxDims] = {};
+  for (long e = 0; e < n; ++e) {
+    long r = e, so = 0, dof = 0;
+    for (int t = c.dst.nd - 1; t >= 0; --t) {
+      idx[t] = r % c.dst.ext[t];
+      r /= c.dst.ext[t];
+      so += idx[t] * c.src.str[t];
+      dof += idx[t] * c.dst.str[t];
+    }
+    const double v = c.alpha * c.src.p[so];
+    c.dst.p[dof] = (c.beta != 0.0) ? v + c.beta * c.dst.p[dof] : v;
+  }
+}
+
+void run_gemm(const ccsd::GemmDesc<double>& g) {
+  for (int b1 = 0; b1 < g.nb1; ++b1)
+    for (int b2 = 0; b2 < g.nb2; ++b2) {
+      const double* A = g.A + b1 * g.sA1 + b2 * g.sA2;
+      const double* B = g.B + b1 * g.sB1 + b2 * g.sB2;
+      double* C = g.C + b1 * g.sC1 + b2 * g.sC2;
+      for (int m = 0; m < g.M; ++m)
+        for (int n = 0; n < g.N; ++n) {
+          double s = 0.0;
+          for (int k = 0; k < g.K; ++k) s += A[m * g.sAm + k * g.sAk] * B[k * g.sBk + n * g.sBn];
+          double* c = C + m * g.sCm + n * g.sCn;
+          *c = g.alpha * s + (g.beta != 0.0 ? g.beta * *c : 0.0);
+        }
+    }
+}
+
+struct CpuExec {
+  Scratch sc;
+  std::vector<double> buf[3];
+  std::string err;
+  long permuted = 0;  // elements copied by planned permutes
+
+  explicit CpuExec(long cap) {
+    for (int k = 0; k < 3; ++k) {
+      buf[k].assign(cap, 0.0);
+      sc.p[k] = buf[k].data();
+      sc.cap[k] = cap;
+    }
+  }
+  int ein(const char* spec, double alpha, const View& A, const View& B, double beta, const View& C) {
+    Plan p;
+    if (!make_plan(spec, alpha, A, B, beta, C, sc, &p, &err)) return -1;
+    for (const Copy& c : p.pre) {
+      run_copy(c);
+      permuted += numel(c.dst);
+    }
+    run_gemm(p.g);
+    for (const Copy& c : p.post) run_copy(c);
+    return 0;
+  }
+  int axpy(const char* spec, double alpha, const View& A, double beta, const View& C) {
+    Copy c;
+    if (!make_axpy(spec, alpha, A, beta, C, &c, &err)) return -1;
+    run_copy(c);
+    return 0;
+  }
+  int tau(const double* t1, const double* t2, double* tau, long o, long v) {
+    for (long i = 0; i < o; ++i)
+      for (long j = 0; j < o; ++j)
+        for (long a = 0; a < v; ++a)
+          for (long b = 0; b < v; ++b) {
+            const long e = ((i * o + j) * v + a) * v + b;
+            tau[e] = t2[e] + t1[i * v + a] * t1[j * v + b];
+          }
+    return 0;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// R1 [o][v], R2 [o][o][v][v] of the closed-shell equations at (t1, t2) for the
+// MO integrals mo [n^4] (correlated occupied [nf, nocc), virtual [nocc, n)).
+// Returns 0, or -1 with the planner's message printed.  *permuted receives
+// the number of elements the planned operand permutes copied.
+int rcpu_rhs(int n, int nocc, int nf, const double* mo, const double* t1, const double* t2, double* R1, double* R2,
+             long* permuted) {
+  const long o = nocc - nf, v = n - nocc;
+  const long t2n = o * o * v * v, ov3 = o * v * v * v;
+  std::vector<double> st[18];
+  const long sizes[18] = {t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, o * v * o * o, o * o * o * v, o * o * o * o,
+                          ov3, ov3, ov3, ov3, ov3, ov3, v * v * v * v};
+  for (int k = 0; k < 18; ++k) st[k].assign(sizes[k], 0.0);
+  RStatic S;
+  double** sp[18] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.ovoo, &S.Y2, &S.oooo,
+                     &S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL, &S.vvvvL};
+  for (int k = 0; k < 18; ++k) *sp[k] = st[k].data();
+  std::vector<long> wn;
+  rccsd_work_sizes(o, v, &wn);
+  std::vector<std::vector<double>> wk(wn.size());
+  for (size_t k = 0; k < wn.size(); ++k) wk[k].assign(wn[k], 0.0);
+  RWork W;
+  double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
+                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u};
+  for (size_t k = 0; k < wn.size(); ++k) *wp[k] = wk[k].data();
+  CpuExec x(t2n + o * o * o * v);
+  std::vector<double> mo_c(mo, mo + (long)n * n * n * n), t1c(t1, t1 + o * v), t2c(t2, t2 + t2n);
+  int rc = rccsd_build_static(x, mo_c.data(), n, nocc, nf, S);
+  if (rc == 0) rc = rccsd_rhs(x, o, v, S, W, t1c.data(), t2c.data(), R1, R2);
+  if (rc != 0) fprintf(stderr, "rcpu_rhs: %s\n", x.err.c_str());
+  if (permuted) *permuted = x.permuted;
+  return rc;
+}
+
+}  // extern "C"
