@@ -569,6 +569,46 @@ __global__ void gemm_tail_reduce_kernel(GemmDesc<T> d, const T* __restrict__ W, 
   }
 }
 
+// gemv over an m-contiguous A (sAm == 1): workgroup (x, y) owns 64
+// consecutive outputs and the k-range y of gridDim.y equal ranges; its 4 waves
+// take every 4th k of the range (one coalesced 64-element read of A per k, two
+// independent sums), and the 4 wave sums are added in a fixed order into
+// partial[y][m] (or, with one range, straight into C).
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_kernel(GemmDesc<T> d, T* __restrict__ partial) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long m = (long)blockIdx.x * 64 + lane;
+  const long chunk = (d.K + gridDim.y - 1) / gridDim.y;
+  const long k0 = blockIdx.y * chunk, k1 = min((long)d.K, k0 + chunk);
+  T s0 = 0, s1 = 0;
+  if (m < d.M) {
+    long k = k0 + wave;
+    for (; k + 4 < k1; k += 8) {
+      s0 += d.A[m + k * d.sAk] * d.B[k * d.sBk];
+      s1 += d.A[m + (k + 4) * d.sAk] * d.B[(k + 4) * d.sBk];
+    }
+    for (; k < k1; k += 4) s0 += d.A[m + k * d.sAk] * d.B[k * d.sBk];
+  }
+  __shared__ T sh[4][64];
+  sh[wave][lane] = s0 + s1;
+  __syncthreads();
+  if (wave == 0 && m < d.M) {
+    const T v = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+    if (gridDim.y == 1) write_out(d, m * d.sCm, v);
+    else partial[blockIdx.y * (long)d.M + m] = v;
+  }
+}
+
+// C[m] from gemv_t_kernel's k-range partials, summed in range order.
+template <typename T>
+__global__ void gemv_t_reduce_kernel(GemmDesc<T> d, const T* __restrict__ partial, int nsplit) {
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < d.M; m += (long)gridDim.x * blockDim.x) {
+    T s = 0;
+    for (int y = 0; y < nsplit; ++y) s += partial[y * (long)d.M + m];
+    write_out(d, m * d.sCm, s);
+  }
+}
+
 // gemv over unit-stride rows with 16-byte alignment (the host checks): lane l
 // loads pairs k0 + 128 j + 2 l, j = 0..3, into four independent sums, so a
 // wavefront moves 2 KB of the row per iteration; the ragged end is read
@@ -731,6 +771,19 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
     return hipGetLastError();
   }
   if (d.N == 1 && nbatch == 1 && force_config < 0) {
+    if (d.sAm == 1 && d.sAk != 1 && d.M >= 64) {
+      // k-ranges of at least 256 so that about 1024 workgroups are in flight
+      const long mb = (d.M + 63) / 64;
+      long ny = std::min<long>(std::max<long>(1, 1024 / mb), std::max<long>(1, d.K / 256));
+      ny = std::min<long>(ny, (long)(ws.bytes / sizeof(T) / std::max(1, d.M)));
+      if (ny < 1 || !ws.ptr) ny = 1;
+      hipLaunchKernelGGL(gemv_t_kernel<T>, dim3((unsigned)mb, (unsigned)ny), dim3(256), 0, stream, d, (T*)ws.ptr);
+      if (ny > 1)
+        hipLaunchKernelGGL(gemv_t_reduce_kernel<T>, dim3((unsigned)std::min<long>((d.M + 255) / 256, 4096)),
+                           dim3(256), 0, stream, d, (const T*)ws.ptr, (int)ny);
+      if (info) *info = GemmLaunchInfo{d.M, 1, 1, (int)ny, -1, 0};
+      return hipGetLastError();
+    }
     const int rows_per_block = 4;
     const bool pairs = d.sAk == 1 && d.sBk == 1 && d.sAm % 2 == 0 && d.K <= (1 << 30) &&
                        reinterpret_cast<uintptr_t>(d.A) % 16 == 0 && reinterpret_cast<uintptr_t>(d.B) % 16 == 0;
@@ -800,7 +853,25 @@ hipError_t gemm(const GemmDesc<T>& d, const GemmWorkspace& ws, hipStream_t strea
       const double units = double(tiles) * sr;
       // rounds on the busiest XCD (each XCD holds 1/8 of the CUs)
       const double per_xcd = xcd_order ? std::ceil(double(tm * tn) / 8) * double(nbatch) * sr : std::ceil(units / 8);
-      const double rounds = std::ceil(per_xcd / (slots / 8));
+      double rounds = std::ceil(per_xcd / (slots / 8));
+      // whole-K tiles of a product with at least half a round of tiles: the
+      // split tail below cuts the last partial round into k-chunks; model the
+      // rounds it leaves (same rule as the launch)
+      if (sr == 1 && nbatch == 1 && !d.reduce_batch && force_split <= 0 && tm * tn >= kNumCus / 2 * g.slots) {
+        const long sx = (long)(kNumCus / 8) * g.slots, xf = (tm * tn / 8) / sx * sx, tl = tm * tn - 8 * xf;
+        if (xf > 0 && tl > 0) {
+          double br = 0.99 * rounds;
+          for (int sp = 2; sp <= 8; ++sp) {
+            int kq = (d.K + sp - 1) / sp;
+            kq = ((kq + g.BK - 1) / g.BK) * g.BK;
+            const int se = (d.K + kq - 1) / kq;
+            if (se < 2 || kq / g.BK < 8) continue;
+            if (double(tl) * se * g.BM * g.BN * sizeof(T) > double(ws.bytes)) continue;
+            br = std::min(br, double(xf) / sx + std::ceil(std::ceil(double(tl * se) / 8) / sx) / se + 0.01 * se);
+          }
+          rounds = std::min(rounds, br);
+        }
+      }
       const double tc = rounds * slots / units * pad * flops / (g.rate * 1e12);
       double t = std::max(tc, t_hbm);
       if (sr > 1 || need_partial_for_batch) t += (sr + 2) * double(part_bytes_per) / 4e12 + 5e-6;

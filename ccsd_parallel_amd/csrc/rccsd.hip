@@ -137,6 +137,68 @@ __global__ void rmp2_kernel(double* amp, const double* ovovT, const double* eo, 
   RGRID_LOOP(e, n) amp[e] = (e < o * v) ? 0.0 : ovovT[e - o * v] / rden(e, eo, ev, o, v);
 }
 
+// Pair decode: p1 index k = q(q+1)/2 + p (p <= q), p0 index k = q(q-1)/2 + p (p < q).
+__device__ __forceinline__ void dec1(long k, long& p, long& q) {
+  q = (long)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+  while ((q + 1) * (q + 2) / 2 <= k) ++q;
+  while (q * (q + 1) / 2 > k) --q;
+  p = k - q * (q + 1) / 2;
+}
+__device__ __forceinline__ void dec0(long k, long& p, long& q) {
+  q = (long)((sqrt(8.0 * (double)k + 1.0) + 1.0) * 0.5);
+  while (q * (q - 1) / 2 > k) --q;
+  while ((q + 1) * q / 2 <= k) ++q;
+  p = k - q * (q - 1) / 2;
+}
+__device__ __forceinline__ long pk1(long p, long q) { return p <= q ? q * (q + 1) / 2 + p : p * (p + 1) / 2 + q; }
+__device__ __forceinline__ long pk0(long p, long q) { return p < q ? q * (q - 1) / 2 + p : p * (p - 1) / 2 + q; }
+
+struct View4 {
+  const double* p;
+  long s0, s1, s2, s3;
+};
+
+// Vp[(ab)][(cd)] = (ac|bd) + (ad|bc) over a <= b, c <= d; Vm = (ac|bd) - (ad|bc)
+// over a < b, c < d; rows of pitch l1 / l0
+__global__ void rvpm_kernel(double* Vp, double* Vm, View4 w, long v, long l1, long l0) {
+  const long n1 = v * (v + 1) / 2;
+  RGRID_LOOP(e, n1 * n1) {
+    long a, b, c, d;
+    dec1(e / n1, a, b);
+    dec1(e % n1, c, d);
+    const double x = w.p[a * w.s0 + c * w.s1 + b * w.s2 + d * w.s3];
+    const double y = w.p[a * w.s0 + d * w.s1 + b * w.s2 + c * w.s3];
+    Vp[(e / n1) * l1 + e % n1] = x + y;
+    if (a < b && c < d) Vm[pk0(a, b) * l0 + pk0(c, d)] = x - y;
+  }
+}
+
+// tp[(ij)][(cd)] = (tau_ij^cd + tau_ij^dc) / 2, halved again on c = d (i <= j, c <= d);
+// tm[(ij)][(cd)] = (tau_ij^cd - tau_ij^dc) / 2 (i < j, c < d); rows of pitch l1 / l0
+__global__ void rladder_pack_kernel(double* tp, double* tm, const double* tau, long o, long v, long l1, long l0) {
+  const long n1 = v * (v + 1) / 2, m1 = o * (o + 1) / 2;
+  RGRID_LOOP(e, m1 * n1) {
+    long i, j, c, d;
+    dec1(e / n1, i, j);
+    dec1(e % n1, c, d);
+    const double* t = tau + (i * o + j) * v * v;
+    const double x = t[c * v + d], y = t[d * v + c];
+    tp[(e / n1) * l1 + e % n1] = 0.5 * (x + y) * (c == d ? 0.5 : 1.0);
+    if (i < j && c < d) tm[pk0(i, j) * l0 + pk0(c, d)] = 0.5 * (x - y);
+  }
+}
+
+// R2[i][j][a][b] += S[(ij)][(ab)] + sgn(i,j) sgn(a,b) A[(ij)][(ab)]
+__global__ void rladder_unpack_kernel(double* R2, const double* Sl, const double* Al, long o, long v) {
+  const long n1 = v * (v + 1) / 2, n0 = v * (v - 1) / 2;
+  RGRID_LOOP(e, o * o * v * v) {
+    const long b = e % v, a = (e / v) % v, j = (e / (v * v)) % o, i = e / (o * v * v);
+    double x = Sl[pk1(i, j) * n1 + pk1(a, b)];
+    if (i != j && a != b) x += ((i < j) == (a < b) ? 1.0 : -1.0) * Al[pk0(i, j) * n0 + pk0(a, b)];
+    R2[e] += x;
+  }
+}
+
 struct GpuExec {
   RCtx* c;
   int copy(const rplan::Copy& cp) {
@@ -196,13 +258,32 @@ struct GpuExec {
     hipLaunchKernelGGL(rtau_kernel, dim3(rgrid(o * o * v * v)), dim3(kThreadsR), 0, c->stream, tau, t1, t2, o, v);
     return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
   }
+  int vpm(const rplan::View& w, double* Vp, double* Vm, long v) {
+    const View4 w4{w.p, w.str[0], w.str[1], w.str[2], w.str[3]};
+    const long n1 = v * (v + 1) / 2;
+    hipLaunchKernelGGL(rvpm_kernel, dim3(rgrid(n1 * n1)), dim3(kThreadsR), 0, c->stream, Vp, Vm, w4, v,
+                       rplan::ldpair(n1), rplan::ldpair(rplan::npair0(v)));
+    return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+  }
+  int ladder_pack(const double* tau, double* tp, double* tm, long o, long v) {
+    const long n = o * (o + 1) / 2 * (v * (v + 1) / 2);
+    hipLaunchKernelGGL(rladder_pack_kernel, dim3(rgrid(n)), dim3(kThreadsR), 0, c->stream, tp, tm, tau, o, v,
+                       rplan::ldpair(rplan::npair1(v)), rplan::ldpair(rplan::npair0(v)));
+    return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+  }
+  int ladder_unpack(const double* Sl, const double* Al, double* R2, long o, long v) {
+    hipLaunchKernelGGL(rladder_unpack_kernel, dim3(rgrid(o * o * v * v)), dim3(kThreadsR), 0, c->stream, R2, Sl, Al,
+                       o, v);
+    return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+  }
 };
 
 // Device bytes of a context (statics, work, amplitude vectors, DIIS, scratch,
 // workspace), and the transient n^4 buffers of the set-up.
 size_t rplan_bytes(long n, long o, long v, int diis_m, bool with_C, size_t* transient) {
   const long t2 = o * o * v * v, ov3 = o * v * v * v, nvec = o * v + t2;
-  long el = 8 * t2 + 2 * o * o * o * v + o * o * o * o + 7 * ov3 + v * v * v * v;  // statics
+  const long n1 = v * (v + 1) / 2, n0 = v * (v - 1) / 2;
+  long el = 8 * t2 + 2 * o * o * o * v + o * o * o * o + 6 * ov3 + n1 * rplan::ldpair(n1) + n0 * rplan::ldpair(n0);
   std::vector<long> wn;
   rplan::rccsd_work_sizes(o, v, &wn);
   for (long x : wn) el += x;
@@ -349,12 +430,13 @@ int rsetup(RCtx* c, const ccsd_rhf_problem* p) {
   TRY(ralloc(c, &S.oooo, o * o * o * o));
   double** s3[] = {&S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL};
   for (double** q : s3) TRY(ralloc(c, q, ov3));
-  TRY(ralloc(c, &S.vvvvL, v * v * v * v));
+  TRY(ralloc(c, &S.Vp, rplan::npair1(v) * rplan::ldpair(rplan::npair1(v))));
+  TRY(ralloc(c, &S.Vm, rplan::npair0(v) * rplan::ldpair(rplan::npair0(v))));
   std::vector<long> wn;
   rplan::rccsd_work_sizes(o, v, &wn);
   rplan::RWork& W = c->W;
   double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
-                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u};
+                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al};
   for (size_t k = 0; k < wn.size(); ++k) TRY(ralloc(c, wp[k], wn[k]));
   c->nvec = o * v + t2;
   TRY(ralloc(c, &c->amp, c->nvec));

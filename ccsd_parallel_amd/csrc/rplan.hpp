@@ -17,6 +17,9 @@
 //   int ein(spec, alpha, A, B, beta, C)   C = alpha A.B + beta C
 //   int axpy(spec, alpha, A, beta, C)     C = alpha perm(A) + beta C
 //   int tau(t1, t2, tau, o, v)            tau = t2 + t1 (x) t1
+//   int vpm(VVVV, Vp, Vm, v)               the pair-packed ladder integrals
+//   int ladder_pack(tau, tp, tm, o, v)     symmetric / antisymmetric tau pairs
+//   int ladder_unpack(Sl, Al, R2, o, v)    R2 += S + A unpacked
 // so that the same sequence runs on the device (rccsd.hip) and, as test
 // infrastructure, on host arrays.
 #pragma once
@@ -420,8 +423,11 @@ struct RStatic {
   double *ovvvR2 = nullptr; // [a][k][c][d] = (kd|ac)
   double *ovvvY = nullptr;  // [k][c][d][a] = 2 (kd|ac) - (kc|ad)
   double *ovvvL = nullptr;  // [k][d][a][c] = 2 (kd|ac) - (kc|ad)
-  // v^4
-  double *vvvvL = nullptr;  // [a][b][c][d] = (ac|bd)
+  // pp ladder over pair-packed indices (a <= b, c <= d / a < b, c < d):
+  // Vp[(ab)][(cd)] = (ac|bd) + (ad|bc), Vm[(ab)][(cd)] = (ac|bd) - (ad|bc);
+  // rows of pitch ldpair(columns)
+  double *Vp = nullptr;     // [v(v+1)/2][ldpair(v(v+1)/2)]
+  double *Vm = nullptr;     // [v(v-1)/2][ldpair(v(v-1)/2)]
 };
 
 // Per-iteration work arrays.
@@ -440,7 +446,27 @@ struct RWork {
   double *Xk = nullptr;     // [k][i][j][a] (o^3 v)
   double *Woooo = nullptr;  // [k][l][i][j]
   double *Fov = nullptr, *Foo = nullptr, *Fvv = nullptr, *Loo = nullptr, *Lvv = nullptr, *u = nullptr;
+  // symmetric / antisymmetric ladder: tp [o(o+1)/2][ldpair(v(v+1)/2)], tm [o(o-1)/2][ldpair(v(v-1)/2)],
+  // results Sl [o(o+1)/2][v(v+1)/2], Al [o(o-1)/2][v(v-1)/2]
+  double *tp = nullptr, *tm = nullptr, *Sl = nullptr, *Al = nullptr;
 };
+
+// Pair counts and pair-packed indices: p1(p, q) = q(q+1)/2 + p for p <= q,
+// p0(p, q) = q(q-1)/2 + p for p < q.
+inline long npair1(long n) { return n * (n + 1) / 2; }
+inline long npair0(long n) { return n * (n - 1) / 2; }
+// Row pitch of the pair-packed ladder arrays: a multiple of 8 elements, so that
+// rows start 64-byte aligned and the GEMM engine can load element pairs.
+inline long ldpair(long n) { return (n + 7) / 8 * 8; }
+
+inline View rows(double* p, long m, long k, long ld) {
+  View w;
+  w.p = p;
+  w.nd = 2;
+  w.ext[0] = m; w.ext[1] = k;
+  w.str[0] = ld; w.str[1] = 1;
+  return w;
+}
 
 // Build every static block from a view of the MO integrals mo[p][q][r][s]
 // (n^4, chemists' notation); correlated occupied orbitals [nf, nocc), virtuals
@@ -490,7 +516,7 @@ int rccsd_build_static(X& x, double* mo, int n, int nocc, int nf, const RStatic&
   RT(x.axpy("kcad->kcda", -1.0, ovvv, 1.0, contig(S.ovvvY, {o, v, v, v})));
   RT(x.axpy("kdac->kdac", 2.0, ovvv, 0.0, contig(S.ovvvL, {o, v, v, v})));
   RT(x.axpy("kcad->kdac", -1.0, ovvv, 1.0, contig(S.ovvvL, {o, v, v, v})));
-  RT(x.axpy("acbd->abcd", 1.0, VVVV, 0.0, contig(S.vvvvL, {v, v, v, v})));
+  RT(x.vpm(VVVV, S.Vp, S.Vm, v));
 #undef RT
   return 0;
 }
@@ -516,7 +542,6 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
   const View ovvvX = contig(S.ovvvX, {o, v, v, v}), ovvvR = contig(S.ovvvR, {v, o, v, v});
   const View ovvvR2 = contig(S.ovvvR2, {v, o, v, v}), ovvvY = contig(S.ovvvY, {o, v, v, v});
   const View ovvvL = contig(S.ovvvL, {o, v, v, v});
-  const View vvvvL = contig(S.vvvvL, {v, v, v, v});
   int rc;
 #define RT(call) \
   if ((rc = (call)) != 0) return rc
@@ -550,7 +575,16 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
   RT(x.ein("kclj,ic->klij", 1.0, ovoo, t1, 1.0, Woooo));
   RT(x.ein("klcd,ijcd->klij", 1.0, ovovP, tau, 1.0, Woooo));
   RT(x.ein("klij,klab->ijab", 1.0, Woooo, tau, 1.0, R2));
-  RT(x.ein("ijcd,abcd->ijab", 1.0, tau, vvvvL, 1.0, R2));
+  // pp ladder sum_cd (ac|bd) tau_ij^cd = S + A over pairs: S_ij^ab = sum_{c<=d} Vp tp
+  // (symmetric in a,b and in i,j), A_ij^ab = sum_{c<d} Vm tm (antisymmetric in
+  // a,b and in i,j); tp carries tau_ij^cc / 2 on the diagonal c = d
+  const long n1 = npair1(v), n0 = npair0(v), l1 = ldpair(n1), l0 = ldpair(n0);
+  RT(x.ladder_pack(W.tau, W.tp, W.tm, o, v));
+  RT(x.ein("pk,qk->pq", 1.0, rows(W.tp, npair1(o), n1, l1), rows(S.Vp, n1, n1, l1), 0.0,
+           contig(W.Sl, {npair1(o), n1})));
+  RT(x.ein("pk,qk->pq", 1.0, rows(W.tm, npair0(o), n0, l0), rows(S.Vm, n0, n0, l0), 0.0,
+           contig(W.Al, {npair0(o), n0})));
+  RT(x.ladder_unpack(W.Sl, W.Al, R2p, o, v));
   // t1 parts of the ladder and of (kc|ai) t: Xk[k][i][j][a] = sum_cd (kd|ac) tau_ij^cd
   //   + sum_c (kc|ia) t_jc + (ia|kj); then -P sum_k Xk t_kb
   RT(x.ein("kacd,ijcd->kija", 1.0, ovvvX, tau, 0.0, Xk));
@@ -600,7 +634,8 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
 inline void rccsd_work_sizes(long o, long v, std::vector<long>* n) {
   const long t2 = o * o * v * v;
   *n = {t2, t2, t2, t2, t2, t2, t2, t2, t2, t2, o * o * v * o, o * o * o * v, o * o * o * o, o * v, o * o, v * v,
-        o * o, v * v, v * v};
+        o * o, v * v, v * v, npair1(o) * ldpair(npair1(v)), npair0(o) * ldpair(npair0(v)), npair1(o) * npair1(v),
+        npair0(o) * npair0(v)};
 }
 
 }  // namespace rplan

@@ -77,6 +77,46 @@ struct CpuExec {
     run_copy(c);
     return 0;
   }
+  static long p1(long p, long q) { return p <= q ? q * (q + 1) / 2 + p : p * (p + 1) / 2 + q; }
+  static long p0(long p, long q) { return p < q ? q * (q - 1) / 2 + p : p * (p - 1) / 2 + q; }
+  int vpm(const View& W, double* Vp, double* Vm, long v) {
+    auto at = [&](long a, long b, long c, long d) {
+      return W.p[a * W.str[0] + b * W.str[1] + c * W.str[2] + d * W.str[3]];
+    };
+    for (long b = 0; b < v; ++b)
+      for (long a = 0; a <= b; ++a)
+        for (long d = 0; d < v; ++d)
+          for (long c = 0; c <= d; ++c) {
+            const double x = at(a, c, b, d), y = at(a, d, b, c);
+            Vp[p1(a, b) * ldpair(npair1(v)) + p1(c, d)] = x + y;
+            if (a < b && c < d) Vm[p0(a, b) * ldpair(npair0(v)) + p0(c, d)] = x - y;
+          }
+    return 0;
+  }
+  int ladder_pack(const double* tau, double* tp, double* tm, long o, long v) {
+    const long n1 = ldpair(npair1(v)), n0 = ldpair(npair0(v));
+    for (long j = 0; j < o; ++j)
+      for (long i = 0; i <= j; ++i)
+        for (long d = 0; d < v; ++d)
+          for (long c = 0; c <= d; ++c) {
+            const double x = tau[((i * o + j) * v + c) * v + d], y = tau[((i * o + j) * v + d) * v + c];
+            tp[p1(i, j) * n1 + p1(c, d)] = 0.5 * (x + y) * (c == d ? 0.5 : 1.0);
+            if (i < j && c < d) tm[p0(i, j) * n0 + p0(c, d)] = 0.5 * (x - y);
+          }
+    return 0;
+  }
+  int ladder_unpack(const double* Sl, const double* Al, double* R2, long o, long v) {
+    const long n1 = v * (v + 1) / 2, n0 = v * (v - 1) / 2;
+    for (long i = 0; i < o; ++i)
+      for (long j = 0; j < o; ++j)
+        for (long a = 0; a < v; ++a)
+          for (long b = 0; b < v; ++b) {
+            double x = Sl[p1(i, j) * n1 + p1(a, b)];
+            if (i != j && a != b) x += ((i < j) == (a < b) ? 1.0 : -1.0) * Al[p0(i, j) * n0 + p0(a, b)];
+            R2[((i * o + j) * v + a) * v + b] += x;
+          }
+    return 0;
+  }
   int tau(const double* t1, const double* t2, double* tau, long o, long v) {
     for (long i = 0; i < o; ++i)
       for (long j = 0; j < o; ++j)
@@ -101,21 +141,21 @@ int rcpu_rhs(int n, int nocc, int nf, const double* mo, const double* t1, const 
              long* permuted) {
   const long o = nocc - nf, v = n - nocc;
   const long t2n = o * o * v * v, ov3 = o * v * v * v;
-  std::vector<double> st[18];
-  const long sizes[18] = {t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, o * v * o * o, o * o * o * v, o * o * o * o,
-                          ov3, ov3, ov3, ov3, ov3, ov3, v * v * v * v};
-  for (int k = 0; k < 18; ++k) st[k].assign(sizes[k], 0.0);
+  std::vector<double> st[19];
+  const long sizes[19] = {t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, o * v * o * o, o * o * o * v, o * o * o * o,
+                          ov3, ov3, ov3, ov3, ov3, ov3, npair1(v) * ldpair(npair1(v)), npair0(v) * ldpair(npair0(v))};
+  for (int k = 0; k < 19; ++k) st[k].assign(sizes[k], 0.0);
   RStatic S;
-  double** sp[18] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.ovoo, &S.Y2, &S.oooo,
-                     &S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL, &S.vvvvL};
-  for (int k = 0; k < 18; ++k) *sp[k] = st[k].data();
+  double** sp[19] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.ovoo, &S.Y2, &S.oooo,
+                     &S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL, &S.Vp, &S.Vm};
+  for (int k = 0; k < 19; ++k) *sp[k] = st[k].data();
   std::vector<long> wn;
   rccsd_work_sizes(o, v, &wn);
   std::vector<std::vector<double>> wk(wn.size());
   for (size_t k = 0; k < wn.size(); ++k) wk[k].assign(wn[k], 0.0);
   RWork W;
   double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
-                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u};
+                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al};
   for (size_t k = 0; k < wn.size(); ++k) *wp[k] = wk[k].data();
   CpuExec x(t2n + o * o * o * v);
   std::vector<double> mo_c(mo, mo + (long)n * n * n * n), t1c(t1, t1 + o * v), t2c(t2, t2 + t2n);

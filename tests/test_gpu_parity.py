@@ -40,7 +40,9 @@ def packed(t2, O, V):
     (190, 257, 4000, 0, 1), (5, 5, 20000, 0, 0), (300, 1, 300, 0, 0),
     # N = 1: pair loads with full 512-element chunks and a ragged end; an odd
     # row pitch (element loads); a transposed A (element loads)
-    (257, 1, 1500, 0, 0), (301, 1, 301, 0, 0), (64, 1, 700, 1, 0)])
+    (257, 1, 1500, 0, 0), (301, 1, 301, 0, 0), (64, 1, 700, 1, 0),
+    # N = 1 over an m-contiguous A of at least 64 rows (column kernel), K not a multiple of 8
+    (1000, 1, 3001, 1, 0), (130, 1, 7, 1, 1)])
 def test_gemm_fp64(M, N, K, ta, tb):
     capi = _ctx()
     rng = np.random.default_rng(M * 7 + N)

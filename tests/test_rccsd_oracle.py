@@ -20,7 +20,7 @@ import pytest
 from oracle import ccsd_ref, rccsd_ref, rhf_spin
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SHAPES = [(8, 3, 0, 5), (9, 4, 1, 6), (7, 2, 0, 7), (10, 5, 2, 8)]
+SHAPES = [(8, 3, 0, 5), (9, 4, 1, 6), (7, 2, 0, 7), (10, 5, 2, 8), (6, 1, 0, 12), (7, 3, 2, 13)]
 
 
 def _amps(I, seed, scale=0.05):
