@@ -160,7 +160,23 @@ def combined_forward_adjoint(capi, O, V, seed, scale, tol=1e-7):
     return res
 
 
-def rhf_setup(capi, O, V, seed=170, cpu_budget_s=5.0):
+def synthetic_rhf(n, nocc, seed=170, sigma=0.015, rank=8):
+    """8-fold symmetric, positive semi-definite (pq|rs) = sum_k L[pq,k] L[rs,k]
+    with L[pq] = L[qp] ~ N(0, sigma^2) (rank 8; sigma keeps the ring operator
+    of the CCSD iteration contractive, so the problem converges), sorted
+    orbital energies in the synthetic generator's ranges, and random orthogonal
+    MO coefficients (AO input, transformed on the device)."""
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal((n, n, rank)) * sigma
+    L = 0.5 * (L + L.transpose(1, 0, 2))
+    Lm = L.reshape(n * n, rank)
+    eri = (Lm @ Lm.T).reshape(n, n, n, n)
+    eps = np.concatenate([np.sort(rng.uniform(-1.5, -0.3, nocc)), np.sort(rng.uniform(0.2, 3.0, n - nocc))])
+    Cm = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    return eps, eri, Cm
+
+
+def rhf_setup(capi, O, V, eps, eri, Cm, cpu_budget_s=5.0):
     """Context set-up from RHF AO data (ccsd_create_rhf) at n = O/2 + V/2
     spatial orbitals: device times of the AO->MO transform (4 GEMMs, 2 n^5
     flops each) and of the packing of the six spin-orbital blocks, the wall
@@ -168,15 +184,6 @@ def rhf_setup(capi, O, V, seed=170, cpu_budget_s=5.0):
     included), and the oracle's quarter transform (numpy BLAS) on a bounded
     slab of the same tensor, extrapolated by flop count."""
     nocc, n = O // 2, O // 2 + V // 2
-    rng = np.random.default_rng(seed)
-    # 8-fold symmetric, positive semi-definite synthetic (pq|rs) = sum_k L[pq,k] L[rs,k], L[pq] = L[qp]
-    K = 8
-    L = rng.standard_normal((n, n, K)) * (0.5 / np.sqrt(K))
-    L = 0.5 * (L + L.transpose(1, 0, 2))
-    Lm = L.reshape(n * n, K)
-    eri = (Lm @ Lm.T).reshape(n, n, n, n)
-    eps = np.concatenate([np.sort(rng.uniform(-1.5, -0.3, nocc)), np.sort(rng.uniform(0.2, 3.0, n - nocc))])
-    Cm = np.linalg.qr(rng.standard_normal((n, n)))[0]
     t0 = time.perf_counter()
     ctx = capi.Context.from_rhf(eps, eri, nocc, mo_coeff=Cm, diis_dim=8)
     wall = time.perf_counter() - t0
@@ -201,6 +208,53 @@ def rhf_setup(capi, O, V, seed=170, cpu_budget_s=5.0):
         "cpu_baseline": {"value": flops / cpu_rate, "unit": "s (AO->MO transform, extrapolated)", "kind": "port",
                          "sample": f"numpy BLAS quarter pass on a {rows} x {n} slab, {reps} reps in {el:.1f}s, "
                                    f"{cpu_rate / 1e9:.1f} GFLOP/s scaled to {flops:.3e} flop"},
+    }
+
+
+def closed_shell_flops(o, v):
+    """GEMM flops of one closed-shell iteration (terms of rplan.hpp's rccsd_rhs
+    above o^2 v^3): six ov x ov x ov ring products, the S/A pp ladder over
+    packed pairs, the t1 part of the ladder (ov x o^2 x v^2), hh ladder and
+    W_oooo, F_oo / F_vv, the two T1 o^2 v^3 terms."""
+    ov, n1, n0 = o * v, v * (v + 1) // 2, v * (v - 1) // 2
+    return (6 * 2.0 * ov ** 3 + 2.0 * (o * (o + 1) // 2) * n1 * n1 + 2.0 * (o * (o - 1) // 2) * n0 * n0
+            + 2.0 * ov * o * o * v * v + 2 * 2.0 * o ** 4 * v * v + 2 * 2.0 * o ** 3 * v ** 2 + 2 * 2.0 * o ** 2 * v ** 3)
+
+
+def rhf_closed_shell(capi, O, V, eps, eri, Cm, tol=1e-7, steps=5):
+    """The RHF workload at n = O/2 + V/2 spatial orbitals through both paths:
+    the closed-shell spin-adapted iteration (ccsd_rhf_*) and the spin-orbital
+    one (ccsd_create_rhf, the O/V spin-orbital shape of the headline), each
+    from the MP2 guess to |dE| < tol: seconds per iteration, time to converge,
+    iterations and the energy difference."""
+    nocc = O // 2
+    o, v = nocc, V // 2
+    t0 = time.perf_counter()
+    cs = capi.RhfContext(eps, eri, nocc, mo_coeff=Cm, diis_dim=8)
+    create = time.perf_counter() - t0
+    cs.iterate(2)
+    t0 = time.perf_counter()
+    cs.iterate(steps)
+    s_iter = (time.perf_counter() - t0) / steps
+    cs.reset()
+    r = cs.solve(tol, 200)
+    cs.close()
+    t0 = time.perf_counter()
+    so = capi.Context.from_rhf(eps, eri, nocc, mo_coeff=Cm, diis_dim=8)
+    create_so = time.perf_counter() - t0
+    rs = so.solve(tol, 200)
+    so.close()
+    fl = closed_shell_flops(o, v)
+    return {
+        "workload": f"RHF input, n={o + v} spatial orbitals ({o} doubly occupied), synthetic rank-8 (pq|rs), AO basis",
+        "closed_shell": {"s_per_iter": s_iter, "tflops_iteration": fl / s_iter / 1e12, "flops_per_iteration": fl,
+                         "create_wall_s": create, "e_corr": r["e_corr"], "iterations": r["iterations"],
+                         "seconds_total": r["seconds_total"]},
+        "spin_orbital": {"s_per_iter": rs["seconds_per_iter"], "create_wall_s": create_so, "e_corr": rs["e_corr"],
+                         "iterations": rs["iterations"], "seconds_total": rs["seconds_total"]},
+        "abs_dE_corr": abs(r["e_corr"] - rs["e_corr"]),
+        "speedup_time_to_converge": rs["seconds_total"] / r["seconds_total"],
+        "tol": tol,
     }
 
 
@@ -311,7 +365,10 @@ def main():
     if not args.no_adjoint:
         out["forward_adjoint_1e-7"] = combined_forward_adjoint(capi, O, V, seed, scale)
     if not args.no_rhf:
-        out["rhf_setup"] = rhf_setup(capi, O, V)
+        eps_r, eri_r, C_r = synthetic_rhf(O // 2 + V // 2, O // 2)
+        out["rhf_setup"] = rhf_setup(capi, O, V, eps_r, eri_r, C_r)
+        out["rhf_closed_shell_1e-7"] = rhf_closed_shell(capi, O, V, eps_r, eri_r, C_r)
+        del eri_r
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(O, V, seed, scale)
     print(json.dumps(out))

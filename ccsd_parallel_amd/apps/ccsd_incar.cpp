@@ -15,6 +15,7 @@
 // Extensions of this build:
 //   Precision <fp64|fp32|bf16x3>   amplitude-update precision before the FP64 sweeps (default fp64)
 //   Checkpoint <path>              amplitude checkpoint / restart file
+//   Spin_Adapted <auto|true|false> closed-shell spin-adapted amplitudes (auto: with fp64)
 //   DIIS <dim>                     DIIS subspace (0 = plain Jacobi as the reference; default 8)
 // Usage: ccsd_incar [INCAR]
 #include <cmath>
@@ -42,7 +43,7 @@ int charge_of(const std::string& sym) {
 bool is_true(const std::string& v) { return v.compare(0, 4, "true") == 0; }
 
 struct Input {
-  std::string basis, method, precision = "fp64", checkpoint;
+  std::string basis, method, precision = "fp64", checkpoint, spin_adapted = "auto";
   double toler = 1e-8;
   bool freeze = false, relax = false, angstrom = false, print = false;
   int diis = 8;
@@ -81,6 +82,7 @@ int parse(const std::string& path, Input& in) {
     else if (key == "print_stuffies") in.print = is_true(val);
     else if (key == "Precision") in.precision = val;
     else if (key == "Checkpoint") in.checkpoint = val;
+    else if (key == "Spin_Adapted") in.spin_adapted = val;
     else if (key == "DIIS") in.diis = std::atoi(val.c_str());
     else if (in_atoms) {
       const int z = charge_of(key);
@@ -150,6 +152,12 @@ int main(int argc, char** argv) {
   }
   o.diis_dim = in.diis;
   cc.set_options(o);
+  if (in.spin_adapted == "true") cc.set_spin_adapted(1);
+  else if (in.spin_adapted == "false") cc.set_spin_adapted(0);
+  else if (in.spin_adapted != "auto") {
+    std::fprintf(stderr, "Spin_Adapted %s: use auto, true or false\n", in.spin_adapted.c_str());
+    return 1;
+  }
   if (!in.checkpoint.empty()) cc.set_checkpoint(in.checkpoint);
   energy = cc.CCSD(in.toler, in.print);
   if (cc.last_status() != CCSD_OK) {
@@ -157,7 +165,8 @@ int main(int argc, char** argv) {
     return 1;
   }
   std::printf("Energy = %.12f med CCSD metoden\n", energy);
-  std::printf("E_HF %.12f E_corr %.12f E_total %.12f iterations %d restarted %d\n", energy - cc.correlation_energy(),
-              cc.correlation_energy(), energy, cc.last_result().iterations, cc.restarted() ? 1 : 0);
+  std::printf("E_HF %.12f E_corr %.12f E_total %.12f iterations %d restarted %d closed_shell %d\n",
+              energy - cc.correlation_energy(), cc.correlation_energy(), energy, cc.last_result().iterations,
+              cc.restarted() ? 1 : 0, cc.closed_shell_used() ? 1 : 0);
   return 0;
 }

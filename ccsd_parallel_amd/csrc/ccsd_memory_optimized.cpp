@@ -1,9 +1,12 @@
 // Drop-in CCSD_Memory_optimized on top of the C ABI (see include/ccsd_memory_optimized.h).
 //
 // The class gathers the RHF data (from the HF object or the direct
-// constructor) and hands it to ccsd_create_rhf, which transforms the
-// integrals to the MO basis and packs the antisymmetrised spin-orbital blocks
-// on the GPU with the reference's convention (ccsd_memory_optimized.cpp:34-54,
+// constructor) and hands it to the device: by default (FP64) to
+// ccsd_rhf_create, which iterates the closed-shell spin-adapted amplitudes
+// (every spin block of the reference's RHF storage follows from them), and for
+// the mixed precisions to ccsd_create_rhf, which transforms the integrals to
+// the MO basis and packs the antisymmetrised spin-orbital blocks on the GPU
+// with the reference's convention (ccsd_memory_optimized.cpp:34-54,
 // :483-485): occupied spin orbitals first, spin orbital P has spatial orbital
 // P/2 and spin P%2, <PQ||RS> = <PQ|RS> - <PQ|SR> with <PQ|RS> = (pr|qs) when
 // the spins match pairwise (the Fill_integ*_2D slices, e.g. :3728, :3870).
@@ -106,33 +109,46 @@ double CCSD_Memory_optimized::CCSD(double toler, bool print_stuff) {
     printf("Number of Virtuals for CCSD: %d\n", V);
   }
   ccsd_rhf_problem p{n_, nocc_, nf, eps_.data(), C_.empty() ? nullptr : C_.data(), eri_.data()};
+  cs_used_ = spin_mode_ == 1 || (spin_mode_ < 0 && o.precision == CCSD_PREC_FP64);
+  // amplitude shapes: closed-shell t1[o*v], t2[o*o*v*v]; spin-orbital t1[O*V], t2[P2(O)*P2(V)]
+  const int o_s = nocc_ - nf, v_s = n_ - nocc_;
+  const size_t n1 = cs_used_ ? (size_t)o_s * v_s : (size_t)O * V;
+  const size_t n2 = cs_used_ ? (size_t)o_s * o_s * v_s * v_s : (size_t)p2(O) * p2(V);
+  const int hdr_now[2] = {cs_used_ ? -o_s : O, cs_used_ ? -v_s : V};
   ccsd_ctx* ctx = nullptr;
-  status_ = ccsd_create_rhf(&p, &o, &ctx);
+  ccsd_rhf_ctx* rctx = nullptr;
+  status_ = cs_used_ ? ccsd_rhf_create(&p, &o, &rctx) : ccsd_create_rhf(&p, &o, &ctx);
   if (status_ != CCSD_OK) {
     fprintf(stderr, "CCSD_Memory_optimized: %s\n", ccsd_strerror(status_));
     return std::nan("");
   }
-  const size_t n1 = (size_t)O * V, n2 = (size_t)p2(O) * p2(V);
+  auto set_amps = [&](const double* t1, const double* t2) {
+    return cs_used_ ? ccsd_rhf_set_amplitudes(rctx, t1, t2) : ccsd_set_amplitudes(ctx, t1, t2);
+  };
+  auto get_amps = [&](double* t1, double* t2) {
+    return cs_used_ ? ccsd_rhf_get_amplitudes(rctx, t1, t2) : ccsd_get_amplitudes(ctx, t1, t2);
+  };
   restarted_ = false;
   if (!checkpoint_.empty()) {
     if (FILE* fp = fopen(checkpoint_.c_str(), "rb")) {
       int hdr[2] = {0, 0};
       std::vector<double> t1(n1), t2(n2);
-      if (fread(hdr, 4, 2, fp) == 2 && hdr[0] == O && hdr[1] == V && fread(t1.data(), 8, n1, fp) == n1 &&
-          fread(t2.data(), 8, n2, fp) == n2 && ccsd_set_amplitudes(ctx, t1.data(), t2.data()) == CCSD_OK)
+      if (fread(hdr, 4, 2, fp) == 2 && hdr[0] == hdr_now[0] && hdr[1] == hdr_now[1] &&
+          fread(t1.data(), 8, n1, fp) == n1 && fread(t2.data(), 8, n2, fp) == n2 &&
+          set_amps(t1.data(), t2.data()) == CCSD_OK)
         restarted_ = true;
       fclose(fp);
       if (print_stuff) printf("Checkpoint %s: %s\n", checkpoint_.c_str(), restarted_ ? "restart" : "not used");
     }
   }
-  status_ = ccsd_solve(ctx, toler, 1000, &result_);  // Itermax = 1000 (:42)
+  // Itermax = 1000 (:42)
+  status_ = cs_used_ ? ccsd_rhf_solve(rctx, toler, 1000, &result_) : ccsd_solve(ctx, toler, 1000, &result_);
   if (!checkpoint_.empty() && (status_ == CCSD_OK || status_ == CCSD_ERR_NOT_CONVERGED)) {
     std::vector<double> t1(n1), t2(n2);
-    if (ccsd_get_amplitudes(ctx, t1.data(), t2.data()) == CCSD_OK) {
+    if (get_amps(t1.data(), t2.data()) == CCSD_OK) {
       const std::string tmp = checkpoint_ + ".tmp";
       if (FILE* fp = fopen(tmp.c_str(), "wb")) {
-        int hdr[2] = {O, V};
-        bool ok = fwrite(hdr, 4, 2, fp) == 2 && fwrite(t1.data(), 8, n1, fp) == n1 &&
+        bool ok = fwrite(hdr_now, 4, 2, fp) == 2 && fwrite(t1.data(), 8, n1, fp) == n1 &&
                   fwrite(t2.data(), 8, n2, fp) == n2;
         ok = (fclose(fp) == 0) && ok;
         if (ok) std::rename(tmp.c_str(), checkpoint_.c_str());
@@ -140,7 +156,8 @@ double CCSD_Memory_optimized::CCSD(double toler, bool print_stuff) {
       }
     }
   }
-  ccsd_destroy(ctx);
+  if (cs_used_) ccsd_rhf_destroy(rctx);
+  else ccsd_destroy(ctx);
   if (status_ != CCSD_OK && status_ != CCSD_ERR_NOT_CONVERGED) return std::nan("");
   e_corr_ = result_.e_corr;
   return e_corr_ + e_hf_;

@@ -288,7 +288,7 @@ size_t rplan_bytes(long n, long o, long v, int diis_m, bool with_C, size_t* tran
   rplan::rccsd_work_sizes(o, v, &wn);
   for (long x : wn) el += x;
   el += 4 * nvec + 2L * diis_m * nvec + diis_m;  // amp, amp_new, err, R; DIIS
-  el += 3 * (t2 + o * o * o * v);                // planner scratch
+  el += 3 * (t2 + o * o * o * v + o * o * o * o);  // planner scratch
   el += o + v + std::max(1L << 16, 1024L * diis_m) + 4096;  // eo, ev, reduction partials and sums
   *transient = (size_t)(with_C ? 2 : 1) * (size_t)n * n * n * n * 8 + (with_C ? (size_t)n * n * 8 : 0);
   return (size_t)el * 8 + kGemmWorkspaceBytes;
@@ -450,7 +450,8 @@ int rsetup(RCtx* c, const ccsd_rhf_problem* p) {
     TRY(ralloc(c, &c->dcoef, c->diis_m));
     c->B.assign((size_t)c->diis_m * c->diis_m, 0.0);
   }
-  const long scap = t2 + o * o * o * v;
+  // planner scratch: any permuted operand is amplitude-, o^3 v- or o^4-sized
+  const long scap = t2 + o * o * o * v + o * o * o * o;
   for (int k = 0; k < 3; ++k) {
     TRY(ralloc(c, &c->sc.p[k], scap));
     c->sc.cap[k] = scap;

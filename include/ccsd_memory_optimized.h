@@ -94,10 +94,20 @@ class CCSD_Memory_optimized {
   const ccsd_result& last_result() const { return result_; }
   int last_status() const { return status_; }
   void set_options(const ccsd_options& o) { opt_ = o; have_opt_ = true; }
+  // Which amplitudes CCSD() iterates.  -1 (default): the closed-shell
+  // spin-adapted amplitudes (ccsd_rhf_create; the reference's RHF spin
+  // blocks, about 10x fewer flops) when the precision is FP64, the
+  // spin-orbital amplitudes (ccsd_create_rhf) for the mixed precisions;
+  // 1: always closed-shell (FP64 only); 0: always spin-orbital.  Both give the
+  // same energy.
+  void set_spin_adapted(int mode) { spin_mode_ = mode; }
+  bool closed_shell_used() const { return cs_used_; }
   // Amplitude checkpoint: when the file exists and holds amplitudes of the
-  // same shape, CCSD() starts from them instead of the MP2 guess; after the
-  // iteration the final amplitudes are written back.  Format (little endian):
-  // int32 O, V; double t1[O*V]; double t2[P2(O)*P2(V)] (the C ABI's layouts).
+  // same shape and kind, CCSD() starts from them instead of the MP2 guess;
+  // after the iteration the final amplitudes are written back.  Format (little
+  // endian), spin-orbital: int32 O, V; double t1[O*V]; double t2[P2(O)*P2(V)];
+  // closed-shell: int32 -o, -v; double t1[o*v]; double t2[o*o*v*v] (the C
+  // ABI's layouts).
   void set_checkpoint(const std::string& path) { checkpoint_ = path; }
   bool restarted() const { return restarted_; }
   // Spatial orbitals left out of the correlation when freeze_core is set
@@ -128,6 +138,8 @@ class CCSD_Memory_optimized {
   bool restarted_ = false;
   ccsd_result result_{};
   int status_ = 0;
+  int spin_mode_ = -1;
+  bool cs_used_ = false;
 };
 
 #endif

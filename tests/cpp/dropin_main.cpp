@@ -1,6 +1,6 @@
 // Drives the drop-in CCSD_Memory_optimized class through both constructors.
 //
-//   dropin_main <input.bin> [rekeyed|prekey]
+//   dropin_main <input.bin> [rekeyed|prekey] [auto|spin|closed]
 //     Input (binary, little endian): int32 n, nocc; double e_hf; double eps[n];
 //     double C[n*n] (C[mu*n+p], AO mu, MO p); double ao[n^4] ((pq|rs) AO,
 //     [((p*n+q)*n+r)*n+s]); double mo[n^4] (same integrals in the MO basis).
@@ -9,7 +9,8 @@
 //     (o1 o2|o3 o4) (hartree_fock_solver.cpp:798-802), then the re-key
 //     temp_Q(o1,o2)(o3,o4) = field_Q(o1,o3)(o2,o4) (:1045-1210) that get_Energy
 //     applies before the CCSD object asks for slices.  "prekey" skips the
-//     re-key (a provider handing out the SCF's own layout).
+//     re-key (a provider handing out the SCF's own layout).  The last
+//     argument selects the amplitudes the class iterates (set_spin_adapted).
 //   dropin_main --frozen Z1 Z2 ...
 //     Prints Freeze_How_Meny_Orbitals() for those nuclear charges with
 //     freeze_core on and off (no device needed).
@@ -98,12 +99,17 @@ int main(int argc, char** argv) {
   vec Z(1);
   Z(0) = 2.0 * nocc;
   mat R(1, 3);
+  int mode = -1;
+  if (argc >= 4 && strcmp(argv[3], "spin") == 0) mode = 0;
+  if (argc >= 4 && strcmp(argv[3], "closed") == 0) mode = 1;
   CCSD_Memory_optimized a(1, Z, R, "synthetic", 2 * nocc, 0, 1, &hf, false);
+  a.set_spin_adapted(mode);
   double e1 = a.CCSD(1e-11, false);
   CCSD_Memory_optimized b(hf.n, nocc, hf.eps.data(), mo.data(), hf.ehf);
+  b.set_spin_adapted(mode);
   double e2 = b.CCSD(1e-11, false);
-  printf("provider_total %.15f provider_corr %.15f status %d iterations %d\n", e1, a.correlation_energy(),
-         a.last_status(), a.last_result().iterations);
+  printf("provider_total %.15f provider_corr %.15f status %d iterations %d closed_shell %d\n", e1,
+         a.correlation_energy(), a.last_status(), a.last_result().iterations, a.closed_shell_used() ? 1 : 0);
   printf("direct_total %.15f direct_corr %.15f status %d\n", e2, b.correlation_energy(), b.last_status());
   return (a.last_status() == 0 && b.last_status() == 0) ? 0 : 1;
 }

@@ -157,7 +157,7 @@ int rcpu_rhs(int n, int nocc, int nf, const double* mo, const double* t1, const 
   double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
                    &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al};
   for (size_t k = 0; k < wn.size(); ++k) *wp[k] = wk[k].data();
-  CpuExec x(t2n + o * o * o * v);
+  CpuExec x(t2n + o * o * o * v + o * o * o * o);  // the device context's scratch size
   std::vector<double> mo_c(mo, mo + (long)n * n * n * n), t1c(t1, t1 + o * v), t2c(t2, t2 + t2n);
   int rc = rccsd_build_static(x, mo_c.data(), n, nocc, nf, S);
   if (rc == 0) rc = rccsd_rhs(x, o, v, S, W, t1c.data(), t2c.data(), R1, R2);

@@ -144,12 +144,14 @@ def test_frozen_core_count_matches_reference(tmp_path):
 
 
 @pytest.mark.gpu
-def test_dropin_class_matches_oracle(tmp_path):
+@pytest.mark.parametrize("mode", ["auto", "spin", "closed"])
+def test_dropin_class_matches_oracle(tmp_path, mode):
     """The class fed by a provider holding its AO integrals as the reference's
     Hartree_Fock_Solver does (SCF fill, then the re-key get_Energy applies)
-    and by the direct MO constructor: both equal the oracle to 1e-10 Eh.  The
-    same provider without the re-key gives another energy (the layout matters
-    and the test sees it)."""
+    and by the direct MO constructor: both equal the oracle to 1e-10 Eh, with
+    the closed-shell iteration (the FP64 default) and with the spin-orbital
+    one.  The same provider without the re-key gives another energy (the
+    layout matters and the test sees it)."""
     n, nocc = 8, 3
     eps, mo = rhf_spin.random_rhf(n, nocc, 7)
     rng = np.random.default_rng(8)
@@ -160,16 +162,17 @@ def test_dropin_class_matches_oracle(tmp_path):
     inp = str(tmp_path / "in.bin")
     _write_input(inp, n, nocc, ehf, eps, C, ao, mo)
     exe = _build_driver(tmp_path)
-    out = subprocess.run([exe, inp], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, inp, "rekeyed", mode], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     vals = dict(zip(out.stdout.split()[0::2], out.stdout.split()[1::2]))
+    assert vals["closed_shell"] == ("0" if mode == "spin" else "1")
     prob = rhf_spin.problem_from_rhf(eps, mo, nocc)
     E, *_ = ccsd_ref.ccsd_energy(prob, toler=1e-12, max_iter=300, diis_dim=8)
     assert abs(float(vals["provider_corr"]) - E) < 1e-10
     assert abs(float(vals["provider_total"]) - (E + ehf)) < 1e-10
     assert abs(float(vals["direct_corr"]) - E) < 1e-10
     assert abs(float(vals["direct_total"]) - (E + ehf)) < 1e-10
-    bad = subprocess.run([exe, inp, "prekey"], capture_output=True, text=True, timeout=120)
+    bad = subprocess.run([exe, inp, "prekey", mode], capture_output=True, text=True, timeout=120)
     vb = dict(zip(bad.stdout.split()[0::2], bad.stdout.split()[1::2]))
     wrong = vb.get("provider_corr", "nan")
     assert not (np.isfinite(float(wrong)) and abs(float(wrong) - E) < 1e-6), (wrong, E)
@@ -225,3 +228,8 @@ def test_incar_checkpoint_restart_and_mixed_precision(tmp_path):
     assert abs(a["E_corr"] - b["E_corr"]) < 1e-10
     m = _incar_run(tmp_path, "INCAR_m", extra="Precision bf16x3")
     assert abs(m["E_corr"] - a["E_corr"]) < 1e-7
+    # FP64 iterates the closed-shell amplitudes, the mixed run the spin-orbital
+    # ones; a spin-orbital FP64 run neither reads nor breaks the closed-shell checkpoint
+    assert a["closed_shell"] == 1 and m["closed_shell"] == 0
+    s = _incar_run(tmp_path, "INCAR_s", extra=f"Checkpoint {ck}\nSpin_Adapted false")
+    assert s["closed_shell"] == 0 and s["restarted"] == 0 and abs(s["E_corr"] - a["E_corr"]) < 1e-10

@@ -26,7 +26,8 @@ def _amps(o, v, seed, scale=0.05):
 
 
 @pytest.mark.parametrize("n,nocc,nf,seed", [(8, 3, 0, 5), (9, 4, 1, 6), (7, 2, 0, 7), (20, 5, 1, 9), (26, 6, 0, 10),
-                                          (6, 1, 0, 12), (7, 3, 2, 13)])
+                                          (6, 1, 0, 12), (7, 3, 2, 13), (7, 5, 0, 14), (12, 8, 1, 15),
+                                          (6, 5, 0, 16), (40, 28, 2, 17)])
 def test_residual_matches_oracle(n, nocc, nf, seed):
     capi = _capi()
     print("case:", n, nocc, nf, seed, flush=True)

@@ -20,7 +20,9 @@ import pytest
 from oracle import ccsd_ref, rccsd_ref, rhf_spin
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SHAPES = [(8, 3, 0, 5), (9, 4, 1, 6), (7, 2, 0, 7), (10, 5, 2, 8), (6, 1, 0, 12), (7, 3, 2, 13)]
+# o < v, o = 1 (no antisymmetric pairs), o > v (STO-3G water is o = 5, v = 2), v = 1
+SHAPES = [(8, 3, 0, 5), (9, 4, 1, 6), (7, 2, 0, 7), (10, 5, 2, 8), (6, 1, 0, 12), (7, 3, 2, 13), (7, 5, 0, 14),
+          (12, 8, 1, 15), (6, 5, 0, 16)]
 
 
 def _amps(I, seed, scale=0.05):
@@ -114,5 +116,6 @@ def test_planner_and_term_sequence_on_host(tmp_path, n, nocc, nf, seed):
     rc = lib.rcpu_rhs(n, nocc, nf, ptr(np.ascontiguousarray(mo)), ptr(t1), ptr(t2), ptr(r1), ptr(r2), C.byref(perm))
     assert rc == 0
     assert np.abs(r1 - R1).max() <= 1e-14 and np.abs(r2 - R2).max() <= 1e-14
-    # only amplitude-sized operands are permuted (no o v^3 or v^4 block)
-    assert perm.value <= 12 * t2.size
+    # only amplitude-, o^3 v- and o^4-sized operands are permuted (no o v^3 or v^4 block)
+    o, v = I.o, I.v
+    assert perm.value <= 12 * t2.size + 4 * o ** 3 * v + 4 * o ** 4
