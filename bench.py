@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--no-lowp", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-rhf", action="store_true")
+    ap.add_argument("--only-rhf", action="store_true",
+                    help="run only the RHF-input workload (set-up, closed-shell and spin-orbital solves)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (--gpus 1): this build has no multi-GPU path; "
@@ -293,6 +295,11 @@ def main():
 
     from ccsd_parallel_amd import capi
     O, V, seed = args.O, args.V, args.seed
+    if args.only_rhf:
+        eps_r, eri_r, C_r = synthetic_rhf(O // 2 + V // 2, O // 2)
+        print(json.dumps({"rhf_setup": rhf_setup(capi, O, V, eps_r, eri_r, C_r, cpu_budget_s=0.5),
+                          "rhf_closed_shell_1e-7": rhf_closed_shell(capi, O, V, eps_r, eri_r, C_r)}))
+        return
     scale = 1.0 / np.sqrt(O * p2(V))
     fl = flops_per_iteration(O, V)
     total_flops = sum(fl.values())
