@@ -104,6 +104,46 @@ __global__ void strided_copy_kernel(double* dst, const double* src, CopyShape sh
   }
 }
 
+// Transposing copy: the source is contiguous along dimension ds and the
+// destination along dimension dd (ds != dd).  A workgroup moves 32 x 32 tiles
+// of the (ds, dd) plane through LDS so that reads and writes are both
+// coalesced; grid z (and a loop) covers the remaining dimensions, whose
+// extents and strides are in sh (with the tile dimensions' extents set to 1).
+__global__ __launch_bounds__(256) void tiled_copy_kernel(double* dst, const double* src, CopyShape sh, long ext_s,
+                                                         long ext_d, long sstr_d, long dstr_s, long nrest,
+                                                         double alpha, double beta) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const long s0 = (long)blockIdx.x * 32, d0 = (long)blockIdx.y * 32;
+  for (long z = blockIdx.z; z < nrest; z += gridDim.z) {
+    long r = z, so = 0, dof = 0;
+#pragma unroll
+    for (int t = rplan::kMaxDims - 1; t >= 0; --t) {
+      if (t >= sh.nd) continue;
+      const long q = r / sh.ext[t], i = r - q * sh.ext[t];
+      r = q;
+      so += i * sh.sstr[t];
+      dof += i * sh.dstr[t];
+    }
+#pragma unroll
+    for (int y = 0; y < 32; y += 8) {
+      const long is = s0 + tx, id = d0 + y + ty;
+      if (is < ext_s && id < ext_d) tile[y + ty][tx] = src[so + is + id * sstr_d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < 32; y += 8) {
+      const long id = d0 + tx, is = s0 + y + ty;
+      if (is < ext_s && id < ext_d) {
+        double* q = dst + dof + id + is * dstr_s;
+        const double x = alpha * tile[tx][y + ty];
+        *q = (beta != 0.0) ? x + beta * *q : x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // tau[i][j][a][b] = t2 + t1[i][a] t1[j][b]
 __global__ void rtau_kernel(double* tau, const double* t1, const double* t2, long o, long v) {
   const long n = o * o * v * v;
@@ -199,19 +239,77 @@ __global__ void rladder_unpack_kernel(double* R2, const double* Sl, const double
   }
 }
 
+// Xp[(ka)][(cd)] = (kd|ac) + (kc|ad) over c <= d, Xm = (kd|ac) - (kc|ad) over c < d,
+// from ovvv[k][d][a][c] = (kd|ac); rows of pitch l1 / l0
+__global__ void rxpm_kernel(double* Xp, double* Xm, const double* ovvv, long o, long v, long l1, long l0) {
+  const long n1 = v * (v + 1) / 2;
+  RGRID_LOOP(e, o * v * n1) {
+    const long ka = e / n1, k = ka / v, a = ka % v;
+    long c, d;
+    dec1(e % n1, c, d);
+    const double* w = ovvv + k * v * v * v;
+    const double x = w[(d * v + a) * v + c], y = w[(c * v + a) * v + d];
+    Xp[ka * l1 + e % n1] = x + y;
+    if (c < d) Xm[ka * l0 + pk0(c, d)] = x - y;
+  }
+}
+
+// Xk[k][i][j][a] = XS[(ka)][(ij)] + sgn(i,j) XA[(ka)][(ij)]
+__global__ void rxk_unpack_kernel(double* Xk, const double* XS, const double* XA, long o, long v) {
+  const long m1 = o * (o + 1) / 2, m0 = o * (o - 1) / 2;
+  RGRID_LOOP(e, o * o * o * v) {
+    const long a = e % v, j = (e / v) % o, i = (e / (v * o)) % o, k = e / (v * o * o);
+    const long ka = k * v + a;
+    double x = XS[ka * m1 + pk1(i, j)];
+    if (i != j) x += (i < j ? 1.0 : -1.0) * XA[ka * m0 + pk0(i, j)];
+    Xk[e] = x;
+  }
+}
+
 struct GpuExec {
   RCtx* c;
   int copy(const rplan::Copy& cp) {
     CopyShape sh;
-    sh.nd = cp.dst.nd;
+    sh.nd = 0;
     long n = 1, smax = 0, dmax = 0;
-    for (int t = 0; t < sh.nd; ++t) {
-      sh.ext[t] = cp.dst.ext[t];
-      sh.sstr[t] = cp.src.str[t];
-      sh.dstr[t] = cp.dst.str[t];
-      n *= sh.ext[t];
-      smax += (sh.ext[t] - 1) * std::labs(sh.sstr[t]);
-      dmax += (sh.ext[t] - 1) * std::labs(sh.dstr[t]);
+    // dimensions of extent 1 dropped; the rest ordered by decreasing
+    // destination stride, so consecutive threads write consecutive elements
+    int ord[rplan::kMaxDims], m = 0;
+    for (int t = 0; t < cp.dst.nd; ++t)
+      if (cp.dst.ext[t] != 1) ord[m++] = t;
+    std::sort(ord, ord + m, [&](int a, int b) { return std::labs(cp.dst.str[a]) > std::labs(cp.dst.str[b]); });
+    for (int k = 0; k < m; ++k) {
+      const int t = ord[k];
+      sh.ext[k] = cp.dst.ext[t];
+      sh.sstr[k] = cp.src.str[t];
+      sh.dstr[k] = cp.dst.str[t];
+      n *= sh.ext[k];
+      smax += (sh.ext[k] - 1) * std::labs(sh.sstr[k]);
+      dmax += (sh.ext[k] - 1) * std::labs(sh.dstr[k]);
+    }
+    sh.nd = m;
+    for (int t = 0; t < cp.dst.nd; ++t)
+      if (cp.dst.ext[t] == 0) return CCSD_OK;
+    // a transpose in the two unit-stride dimensions: tiled through LDS
+    int ds = -1, dd = -1;
+    for (int k = 0; k < m; ++k) {
+      if (sh.sstr[k] == 1 && ds < 0) ds = k;
+      if (sh.dstr[k] == 1 && dd < 0) dd = k;
+    }
+    if (m >= 2 && ds >= 0 && dd >= 0 && ds != dd && sh.ext[ds] >= 8 && sh.ext[dd] >= 8) {
+      CopyShape rest = sh;
+      const long es = sh.ext[ds], ed = sh.ext[dd], sd = sh.sstr[dd], dsd = sh.dstr[ds];
+      rest.ext[ds] = rest.ext[dd] = 1;
+      const long nrest = n / (es * ed);
+      dim3 grid((unsigned)((es + 31) / 32), (unsigned)((ed + 31) / 32), (unsigned)std::min<long>(nrest, 65535));
+      hipLaunchKernelGGL(tiled_copy_kernel, grid, dim3(256), 0, c->stream, cp.dst.p, cp.src.p, rest, es, ed, sd, dsd,
+                         nrest, cp.alpha, cp.beta);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        c->last_error = hipGetErrorString(e);
+        return CCSD_ERR_HIP;
+      }
+      return CCSD_OK;
     }
     if (n == 0) return CCSD_OK;
     const bool narrow = n < (1L << 31) && smax < (1L << 31) && dmax < (1L << 31);
@@ -271,6 +369,17 @@ struct GpuExec {
                        rplan::ldpair(rplan::npair1(v)), rplan::ldpair(rplan::npair0(v)));
     return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
   }
+  int xpm(const rplan::View& w, double* Xp, double* Xm, long o, long v) {
+    const long n = o * v * rplan::npair1(v);
+    hipLaunchKernelGGL(rxpm_kernel, dim3(rgrid(n)), dim3(kThreadsR), 0, c->stream, Xp, Xm, w.p, o, v,
+                       rplan::ldpair(rplan::npair1(v)), rplan::ldpair(rplan::npair0(v)));
+    return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+  }
+  int xk_unpack(const double* XS, const double* XA, double* Xk, long o, long v) {
+    hipLaunchKernelGGL(rxk_unpack_kernel, dim3(rgrid(o * o * o * v)), dim3(kThreadsR), 0, c->stream, Xk, XS, XA, o,
+                       v);
+    return hipGetLastError() == hipSuccess ? CCSD_OK : CCSD_ERR_HIP;
+  }
   int ladder_unpack(const double* Sl, const double* Al, double* R2, long o, long v) {
     hipLaunchKernelGGL(rladder_unpack_kernel, dim3(rgrid(o * o * v * v)), dim3(kThreadsR), 0, c->stream, R2, Sl, Al,
                        o, v);
@@ -283,7 +392,8 @@ struct GpuExec {
 size_t rplan_bytes(long n, long o, long v, int diis_m, bool with_C, size_t* transient) {
   const long t2 = o * o * v * v, ov3 = o * v * v * v, nvec = o * v + t2;
   const long n1 = v * (v + 1) / 2, n0 = v * (v - 1) / 2;
-  long el = 8 * t2 + 2 * o * o * o * v + o * o * o * o + 6 * ov3 + n1 * rplan::ldpair(n1) + n0 * rplan::ldpair(n0);
+  long el = 10 * t2 + 2 * o * o * o * v + o * o * o * o + 5 * ov3 + (n1 + o * v) * rplan::ldpair(n1) +
+            (n0 + o * v) * rplan::ldpair(n0);  // statics
   std::vector<long> wn;
   rplan::rccsd_work_sizes(o, v, &wn);
   for (long x : wn) el += x;
@@ -423,20 +533,22 @@ int rsetup(RCtx* c, const ccsd_rhf_problem* p) {
   HIPTRY(hipMalloc(&c->ws.ptr, kGemmWorkspaceBytes));
   c->ws.bytes = kGemmWorkspaceBytes;
   rplan::RStatic& S = c->S;
-  double** st[] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT};
+  double** st[] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.oovvI, &S.ovovK};
   for (double** q : st) TRY(ralloc(c, q, t2));
   TRY(ralloc(c, &S.ovoo, o * v * o * o));
   TRY(ralloc(c, &S.Y2, o * o * o * v));
   TRY(ralloc(c, &S.oooo, o * o * o * o));
-  double** s3[] = {&S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL};
+  double** s3[] = {&S.ovvv, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL};
   for (double** q : s3) TRY(ralloc(c, q, ov3));
   TRY(ralloc(c, &S.Vp, rplan::npair1(v) * rplan::ldpair(rplan::npair1(v))));
   TRY(ralloc(c, &S.Vm, rplan::npair0(v) * rplan::ldpair(rplan::npair0(v))));
+  TRY(ralloc(c, &S.Xp, o * v * rplan::ldpair(rplan::npair1(v))));
+  TRY(ralloc(c, &S.Xm, o * v * rplan::ldpair(rplan::npair0(v))));
   std::vector<long> wn;
   rplan::rccsd_work_sizes(o, v, &wn);
   rplan::RWork& W = c->W;
   double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
-                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al};
+                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al, &W.XS, &W.XA};
   for (size_t k = 0; k < wn.size(); ++k) TRY(ralloc(c, wp[k], wn[k]));
   c->nvec = o * v + t2;
   TRY(ralloc(c, &c->amp, c->nvec));

@@ -20,6 +20,8 @@
 //   int vpm(VVVV, Vp, Vm, v)               the pair-packed ladder integrals
 //   int ladder_pack(tau, tp, tm, o, v)     symmetric / antisymmetric tau pairs
 //   int ladder_unpack(Sl, Al, R2, o, v)    R2 += S + A unpacked
+//   int xpm(ovvv, Xp, Xm, o, v)            the pair-packed ladder-t1 integrals
+//   int xk_unpack(XS, XA, Xk, o, v)        Xk = XS + sgn XA unpacked
 // so that the same sequence runs on the device (rccsd.hip) and, as test
 // infrastructure, on host arrays.
 #pragma once
@@ -412,13 +414,18 @@ struct RStatic {
   double *M1 = nullptr;     // [l][d][k][c] = (ld|kc) - 1/2 (lc|kd)
   double *ovovx = nullptr;  // [l][d][k][c] = (lc|kd)
   double *ovovT = nullptr;  // [i][j][a][b] = (ia|jb)
+  double *oovvI = nullptr;  // [i][a][k][c] = (ki|ac)
+  double *ovovK = nullptr;  // [k][i][a][c] = (kc|ia)
   // o^3 v, o^4
   double *ovoo = nullptr;   // [l][c][k][i] = (lc|ki)
   double *Y2 = nullptr;     // [i][l][k][c] = -2 (lc|ki) + (kc|li)
   double *oooo = nullptr;   // [k][i][l][j] = (ki|lj)
   // o v^3
   double *ovvv = nullptr;   // [k][d][a][c] = (kd|ac)
-  double *ovvvX = nullptr;  // [k][a][c][d] = (kd|ac)
+  // t1 part of the ladder over packed pairs: Xp[(ka)][(cd)] = (kd|ac) + (kc|ad)
+  // (c <= d), Xm[(ka)][(cd)] = (kd|ac) - (kc|ad) (c < d); rows of pitch ldpair
+  double *Xp = nullptr;     // [o v][ldpair(v(v+1)/2)]
+  double *Xm = nullptr;     // [o v][ldpair(v(v-1)/2)]
   double *ovvvR = nullptr;  // [a][k][c][d] = (kc|ad)
   double *ovvvR2 = nullptr; // [a][k][c][d] = (kd|ac)
   double *ovvvY = nullptr;  // [k][c][d][a] = 2 (kd|ac) - (kc|ad)
@@ -449,6 +456,8 @@ struct RWork {
   // symmetric / antisymmetric ladder: tp [o(o+1)/2][ldpair(v(v+1)/2)], tm [o(o-1)/2][ldpair(v(v-1)/2)],
   // results Sl [o(o+1)/2][v(v+1)/2], Al [o(o-1)/2][v(v-1)/2]
   double *tp = nullptr, *tm = nullptr, *Sl = nullptr, *Al = nullptr;
+  // Xk over pairs: XS [o v][o(o+1)/2], XA [o v][o(o-1)/2]
+  double *XS = nullptr, *XA = nullptr;
 };
 
 // Pair counts and pair-packed indices: p1(p, q) = q(q+1)/2 + p for p <= q,
@@ -509,7 +518,9 @@ int rccsd_build_static(X& x, double* mo, int n, int nocc, int nf, const RStatic&
   RT(x.axpy("kcli->ilkc", 1.0, ovoo, 1.0, contig(S.Y2, {o, o, o, v})));
   RT(x.axpy("kilj->kilj", 1.0, OOOO, 0.0, contig(S.oooo, {o, o, o, o})));
   RT(x.axpy("kdac->kdac", 1.0, OVVV, 0.0, ovvv));
-  RT(x.axpy("kdac->kacd", 1.0, ovvv, 0.0, contig(S.ovvvX, {o, v, v, v})));
+  RT(x.xpm(ovvv, S.Xp, S.Xm, o, v));
+  RT(x.axpy("kiac->iakc", 1.0, OOVV, 0.0, contig(S.oovvI, {o, v, o, v})));
+  RT(x.axpy("kcia->kiac", 1.0, ovov, 0.0, contig(S.ovovK, {o, o, v, v})));
   RT(x.axpy("kcad->akcd", 1.0, ovvv, 0.0, contig(S.ovvvR, {v, o, v, v})));
   RT(x.axpy("kdac->akcd", 1.0, ovvv, 0.0, contig(S.ovvvR2, {v, o, v, v})));
   RT(x.axpy("kdac->kcda", 2.0, ovvv, 0.0, contig(S.ovvvY, {o, v, v, v})));
@@ -539,7 +550,8 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
   const View M1 = contig(S.M1, {o, v, o, v}), ovovx = contig(S.ovovx, {o, v, o, v});
   const View ovovT = contig(S.ovovT, {o, o, v, v}), ovoo = contig(S.ovoo, {o, v, o, o});
   const View Y2 = contig(S.Y2, {o, o, o, v}), oooo = contig(S.oooo, {o, o, o, o}), ovvv = contig(S.ovvv, {o, v, v, v});
-  const View ovvvX = contig(S.ovvvX, {o, v, v, v}), ovvvR = contig(S.ovvvR, {v, o, v, v});
+  const View ovvvR = contig(S.ovvvR, {v, o, v, v});
+  const View oovvI = contig(S.oovvI, {o, v, o, v}), ovovK = contig(S.ovovK, {o, o, v, v});
   const View ovvvR2 = contig(S.ovvvR2, {v, o, v, v}), ovvvY = contig(S.ovvvY, {o, v, v, v});
   const View ovvvL = contig(S.ovvvL, {o, v, v, v});
   int rc;
@@ -565,7 +577,7 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
   RT(x.ein("kc,ka->ca", 1.0, Fov, t1, 0.0, u));
   RT(x.ein("ic,ca->ia", 1.0, t1, u, 1.0, R1));
   RT(x.ein("kcia,kc->ia", 2.0, ovov, t1, 1.0, R1));
-  RT(x.ein("kiac,kc->ia", -1.0, oovv, t1, 1.0, R1));
+  RT(x.ein("iakc,kc->ia", -1.0, oovvI, t1, 1.0, R1));
   RT(x.ein("ikcd,kcda->ia", 1.0, tau, ovvvY, 1.0, R1));
   RT(x.ein("ilkc,lkca->ia", 1.0, Y2, tau, 1.0, R1));      // tau[k][l][a][c] = tau[l][k][c][a]
   // -- T2: integrals, hh ladder, pp ladder
@@ -586,9 +598,14 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
            contig(W.Al, {npair0(o), n0})));
   RT(x.ladder_unpack(W.Sl, W.Al, R2p, o, v));
   // t1 parts of the ladder and of (kc|ai) t: Xk[k][i][j][a] = sum_cd (kd|ac) tau_ij^cd
-  //   + sum_c (kc|ia) t_jc + (ia|kj); then -P sum_k Xk t_kb
-  RT(x.ein("kacd,ijcd->kija", 1.0, ovvvX, tau, 0.0, Xk));
-  RT(x.ein("kcia,jc->kija", 1.0, ovov, t1, 1.0, Xk));
+  //   + sum_c (kc|ia) t_jc + (ia|kj); then -P sum_k Xk t_kb.  The tau term runs
+  //   over the ladder's packed pairs: XS = Xp tp^T (i <= j), XA = Xm tm^T (i < j)
+  RT(x.ein("qk,pk->qp", 1.0, rows(S.Xp, o * v, n1, l1), rows(W.tp, npair1(o), n1, l1), 0.0,
+           contig(W.XS, {o * v, npair1(o)})));
+  RT(x.ein("qk,pk->qp", 1.0, rows(S.Xm, o * v, n0, l0), rows(W.tm, npair0(o), n0, l0), 0.0,
+           contig(W.XA, {o * v, npair0(o)})));
+  RT(x.xk_unpack(W.XS, W.XA, W.Xk, o, v));
+  RT(x.ein("kiac,jc->kija", 1.0, ovovK, t1, 1.0, Xk));
   RT(x.axpy("iakj->kija", 1.0, ovoo, 1.0, Xk));
   RT(x.ein("kija,kb->ijab", 1.0, Xk, t1, 0.0, Y));
   RT(x.axpy("ijab->ijab", -1.0, Y, 1.0, R2));
@@ -635,7 +652,7 @@ inline void rccsd_work_sizes(long o, long v, std::vector<long>* n) {
   const long t2 = o * o * v * v;
   *n = {t2, t2, t2, t2, t2, t2, t2, t2, t2, t2, o * o * v * o, o * o * o * v, o * o * o * o, o * v, o * o, v * v,
         o * o, v * v, v * v, npair1(o) * ldpair(npair1(v)), npair0(o) * ldpair(npair0(v)), npair1(o) * npair1(v),
-        npair0(o) * npair0(v)};
+        npair0(o) * npair0(v), o * v * npair1(o), o * v * npair0(o)};
 }
 
 }  // namespace rplan

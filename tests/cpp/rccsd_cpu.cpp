@@ -117,6 +117,31 @@ struct CpuExec {
           }
     return 0;
   }
+  int xpm(const View& w, double* Xp, double* Xm, long o, long v) {
+    const long l1 = ldpair(npair1(v)), l0 = ldpair(npair0(v));
+    auto at = [&](long k, long d, long a, long c) { return w.p[k * w.str[0] + d * w.str[1] + a * w.str[2] + c * w.str[3]]; };
+    for (long k = 0; k < o; ++k)
+      for (long a = 0; a < v; ++a)
+        for (long d = 0; d < v; ++d)
+          for (long c = 0; c <= d; ++c) {
+            const double x = at(k, d, a, c), y = at(k, c, a, d);
+            Xp[(k * v + a) * l1 + p1(c, d)] = x + y;
+            if (c < d) Xm[(k * v + a) * l0 + p0(c, d)] = x - y;
+          }
+    return 0;
+  }
+  int xk_unpack(const double* XS, const double* XA, double* Xk, long o, long v) {
+    const long m1 = npair1(o), m0 = npair0(o);
+    for (long k = 0; k < o; ++k)
+      for (long i = 0; i < o; ++i)
+        for (long j = 0; j < o; ++j)
+          for (long a = 0; a < v; ++a) {
+            double x = XS[(k * v + a) * m1 + p1(i, j)];
+            if (i != j) x += (i < j ? 1.0 : -1.0) * XA[(k * v + a) * m0 + p0(i, j)];
+            Xk[((k * o + i) * o + j) * v + a] = x;
+          }
+    return 0;
+  }
   int tau(const double* t1, const double* t2, double* tau, long o, long v) {
     for (long i = 0; i < o; ++i)
       for (long j = 0; j < o; ++j)
@@ -141,21 +166,23 @@ int rcpu_rhs(int n, int nocc, int nf, const double* mo, const double* t1, const 
              long* permuted) {
   const long o = nocc - nf, v = n - nocc;
   const long t2n = o * o * v * v, ov3 = o * v * v * v;
-  std::vector<double> st[19];
-  const long sizes[19] = {t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, o * v * o * o, o * o * o * v, o * o * o * o,
-                          ov3, ov3, ov3, ov3, ov3, ov3, npair1(v) * ldpair(npair1(v)), npair0(v) * ldpair(npair0(v))};
-  for (int k = 0; k < 19; ++k) st[k].assign(sizes[k], 0.0);
+  std::vector<double> st[22];
+  const long sizes[22] = {t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, t2n, o * v * o * o, o * o * o * v,
+                          o * o * o * o, ov3, ov3, ov3, ov3, ov3, npair1(v) * ldpair(npair1(v)),
+                          npair0(v) * ldpair(npair0(v)), o * v * ldpair(npair1(v)), o * v * ldpair(npair0(v))};
+  for (int k = 0; k < 22; ++k) st[k].assign(sizes[k], 0.0);
   RStatic S;
-  double** sp[19] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.ovoo, &S.Y2, &S.oooo,
-                     &S.ovvv, &S.ovvvX, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL, &S.Vp, &S.Vm};
-  for (int k = 0; k < 19; ++k) *sp[k] = st[k].data();
+  double** sp[22] = {&S.ovov, &S.Lov, &S.Lt, &S.oovv, &S.ovovP, &S.M1, &S.ovovx, &S.ovovT, &S.oovvI, &S.ovovK,
+                     &S.ovoo, &S.Y2, &S.oooo, &S.ovvv, &S.ovvvR, &S.ovvvR2, &S.ovvvY, &S.ovvvL, &S.Vp, &S.Vm,
+                     &S.Xp, &S.Xm};
+  for (int k = 0; k < 22; ++k) *sp[k] = st[k].data();
   std::vector<long> wn;
   rccsd_work_sizes(o, v, &wn);
   std::vector<std::vector<double>> wk(wn.size());
   for (size_t k = 0; k < wn.size(); ++k) wk[k].assign(wn[k], 0.0);
   RWork W;
   double** wp[] = {&W.tau, &W.Tr, &W.Tx, &W.Z, &W.Wr1, &W.Wr2, &W.W12, &W.tA, &W.tB, &W.Y, &W.H, &W.Xk, &W.Woooo,
-                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al};
+                   &W.Fov, &W.Foo, &W.Fvv, &W.Loo, &W.Lvv, &W.u, &W.tp, &W.tm, &W.Sl, &W.Al, &W.XS, &W.XA};
   for (size_t k = 0; k < wn.size(); ++k) *wp[k] = wk[k].data();
   CpuExec x(t2n + o * o * o * v + o * o * o * o);  // the device context's scratch size
   std::vector<double> mo_c(mo, mo + (long)n * n * n * n), t1c(t1, t1 + o * v), t2c(t2, t2 + t2n);
