@@ -208,7 +208,8 @@ inline bool batch_runs(const std::vector<char>& bl, const Lab* labs, const View*
 
 }  // namespace detail
 
-// Plan C = alpha A.B + beta C.  On failure returns false with *err set.
+// Plan C = alpha A.B + beta C (C must not alias A or B).  On failure returns
+// false with *err set.
 inline bool make_plan(const char* spec, double alpha, const View& A, const View& B, double beta, const View& C,
                       const Scratch& sc, Plan* out, std::string* err) {
   using namespace detail;
@@ -305,8 +306,6 @@ inline bool make_plan(const char* spec, double alpha, const View& A, const View&
   ord[0] = bl; ord[0].insert(ord[0].end(), ml.begin(), ml.end()); ord[0].insert(ord[0].end(), kl.begin(), kl.end());
   ord[1] = bl; ord[1].insert(ord[1].end(), kl.begin(), kl.end()); ord[1].insert(ord[1].end(), nl.begin(), nl.end());
   ord[2] = bl; ord[2].insert(ord[2].end(), ml.begin(), ml.end()); ord[2].insert(ord[2].end(), nl.begin(), nl.end());
-  View orig_c = vs[2];
-  Lab orig_lc = labs[2];
   for (int k = 0; k < 3; ++k) {
     if (!need[k]) continue;
     const long n = numel(vs[k]);
@@ -327,8 +326,6 @@ inline bool make_plan(const char* spec, double alpha, const View& A, const View&
     vs[k] = dst;
     set_labels(&labs[k], ord[k]);
   }
-  (void)orig_c;
-  (void)orig_lc;
   GemmDesc<double> g;
   long s;
   g.M = (int)extent_of(ml, labs, vs, 3);

@@ -193,4 +193,26 @@ int rcpu_rhs(int n, int nocc, int nf, const double* mo, const double* t1, const 
   return rc;
 }
 
+// One contraction C = alpha A.B + beta C through the planner and the host
+// executor; each operand is a strided view (nd, ext[nd], str[nd]) of its
+// array.  Returns 0, or -1 with the planner's message in err (cap bytes).
+int rcpu_ein(const char* spec, double alpha, double* A, int na, const long* ea, const long* sa, double* B, int nb,
+             const long* eb, const long* sb, double beta, double* Cp, int nc, const long* ec, const long* scn,
+             long scratch, char* err, int cap) {
+  auto view = [](double* p, int nd, const long* e, const long* st) {
+    View w;
+    w.p = p;
+    w.nd = nd;
+    for (int t = 0; t < nd; ++t) {
+      w.ext[t] = e[t];
+      w.str[t] = st[t];
+    }
+    return w;
+  };
+  CpuExec x(scratch);
+  const int rc = x.ein(spec, alpha, view(A, na, ea, sa), view(B, nb, eb, sb), beta, view(Cp, nc, ec, scn));
+  if (rc != 0 && err && cap > 0) snprintf(err, cap, "%s", x.err.c_str());
+  return rc;
+}
+
 }  // extern "C"

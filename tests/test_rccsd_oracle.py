@@ -119,3 +119,86 @@ def test_planner_and_term_sequence_on_host(tmp_path, n, nocc, nf, seed):
     # only amplitude-, o^3 v- and o^4-sized operands are permuted (no o v^3 or v^4 block)
     o, v = I.o, I.v
     assert perm.value <= 12 * t2.size + 4 * o ** 3 * v + 4 * o ** 4
+
+
+# ------------------------------------------------------------- the planner alone
+
+def _ein_lib(tmp_path):
+    lib = _host_executor(tmp_path)
+    lp = C.POINTER(C.c_long)
+    dp = C.POINTER(C.c_double)
+    lib.rcpu_ein.argtypes = [C.c_char_p, C.c_double, dp, C.c_int, lp, lp, dp, C.c_int, lp, lp, C.c_double, dp,
+                             C.c_int, lp, lp, C.c_long, C.c_char_p, C.c_int]
+    return lib
+
+
+def _ein(lib, spec, alpha, A, B, beta, Cm, scratch=1 << 16):
+    """Run one planned contraction on strided numpy views (any strides)."""
+    def parts(x):
+        base = x.base if x.base is not None else x
+        e = np.array(x.shape, dtype=np.int64)
+        s = np.array([st // 8 for st in x.strides], dtype=np.int64)
+        p = C.cast(x.__array_interface__["data"][0], C.POINTER(C.c_double))
+        return p, x.ndim, e.ctypes.data_as(C.POINTER(C.c_long)), s.ctypes.data_as(C.POINTER(C.c_long)), (e, s, base)
+    pa, pb, pc = parts(A), parts(B), parts(Cm)
+    err = C.create_string_buffer(256)
+    rc = lib.rcpu_ein(spec.encode(), alpha, *pa[:4], *pb[:4], beta, *pc[:4], scratch, err, 256)
+    return rc, err.value.decode()
+
+
+@pytest.mark.parametrize("spec,shapes", [
+    ("ik,kj->ij", ((5, 7), (7, 3))),                    # plain GEMM
+    ("ki,kj->ji", ((7, 5), (7, 3))),                    # transposed A and C (C permuted)
+    ("bik,bkj->bij", ((4, 5, 6), (4, 6, 3))),           # one batch run
+    ("abik,abkj->abij", ((2, 3, 5, 4), (2, 3, 4, 6))),  # two batch labels collapsing to one run
+    ("aikb,bkj->bija", ((3, 5, 4, 2), (2, 4, 6))),      # batch labels in different places: permutes
+    ("ia,jb->ijab", ((4, 3), (5, 2))),                  # outer product (K empty)
+    ("iakc,kc->ia", ((3, 4, 5, 6), (5, 6))),            # GEMV (N empty)
+    ("kc,iakc->ia", ((5, 6), (3, 4, 5, 6))),            # M empty
+    ("kcld,ilcd->ki", ((3, 4, 5, 4), (2, 5, 4, 4))),    # K over three labels
+])
+def test_planner_matches_einsum(tmp_path, spec, shapes):
+    lib = _ein_lib(tmp_path)
+    rng = np.random.default_rng(len(spec))
+    A = rng.standard_normal(shapes[0])
+    B = rng.standard_normal(shapes[1])
+    out_shape = np.einsum(spec, A, B).shape
+    C0 = rng.standard_normal(out_shape)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 1.5)):
+        Cm = C0.copy()
+        rc, err = _ein(lib, spec, alpha, A, B, beta, Cm)
+        assert rc == 0, err
+        ref = alpha * np.einsum(spec, A, B) + beta * C0
+        assert np.abs(Cm - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), (spec, alpha, beta)
+
+
+def test_planner_strided_views_and_errors(tmp_path):
+    lib = _ein_lib(tmp_path)
+    rng = np.random.default_rng(3)
+    # strided input views (a column slice, a transposed view) and a strided output view
+    big = rng.standard_normal((6, 11))
+    A = big[:, 2:9]                          # (6, 7), row stride 11
+    B = rng.standard_normal((3, 7)).T        # (7, 3), transposed
+    Cbig = rng.standard_normal((6, 8))
+    C0 = Cbig.copy()
+    Cv = Cbig[:, 1:4]                        # (6, 3), row stride 8
+    rc, err = _ein(lib, "ik,kj->ij", 2.0, A, B, 1.0, Cv)
+    assert rc == 0, err
+    ref = C0.copy()
+    ref[:, 1:4] = 2.0 * A @ B + C0[:, 1:4]
+    assert np.abs(Cbig - ref).max() < 1e-13     # columns outside the view untouched
+    # planner errors, no arithmetic
+    X = rng.standard_normal((3, 4))
+    Y = rng.standard_normal((4, 5))
+    Z = np.zeros((3, 5))
+    for bad in ("ik,kj->ijz", "ik,kk->ij", "ik,kj,->ij", "ix,kj->ij"):
+        rc, err = _ein(lib, bad, 1.0, X, Y, 0.0, Z)
+        assert rc != 0 and err, bad
+    rc, err = _ein(lib, "ik,jk->ij", 1.0, X, np.zeros((5, 5)), 0.0, Z)   # k: 4 against 5
+    assert rc != 0 and err
+    # K labels in different orders in A and B cannot share one stride: a
+    # permute is needed, and a 4-element scratch cannot hold it
+    A = rng.standard_normal((3, 4, 5, 4))
+    B = rng.standard_normal((2, 5, 4, 4))
+    rc, err = _ein(lib, "kcld,ilcd->ki", 1.0, A, B, 0.0, np.zeros((3, 2)), scratch=4)
+    assert rc != 0 and "scratch" in err
