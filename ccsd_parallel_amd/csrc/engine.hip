@@ -281,7 +281,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (lowp) {
     TRY(lowprec_allocate(c, prec));
     L = c->lp;
-    k_unpack_occ_f32(L->T2u_f, L->tauu_f, t2, c->tau, d, s);
+    k_unpack_occ_f32(L->T2u_f, nullptr, t2, c->tau, d, s);
     k_unpack_vir_ring_lp(L->Tv_f, lowprec_ring_out(c, prec), t2, d, s);
     k_to_f32(L->t1_f, t1, ov, s);
     TRY(lowprec_amplitudes(c, prec));
@@ -327,15 +327,19 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     k_to_f32(L->W3u_f, c->W3u, O * O * O * V, s);
   }
 
-  // -- W_mbij (layout [ij][(m,b)]) = <mb||ij> + 1/2 tau <mb||ef> + P(ij) t_i^e <mb||ej>
-  k_copy(c->W2, c->ooov, po * ov, s);
+  // -- W_mbij (layout [ij][(m,b)]) = <mb||ij> + 1/2 tau <mb||ef> + P(ij) t_i^e <mb||ej>.
+  // The product P = 1/2 tau <mb||ef> lands in W2 alone first: the T1 term
+  // -1/2 tau_im^ef <ma||ef> is its partial trace over m (k_r1_from_w2), so that
+  // term costs no second pass over <ma||ef>.
+  k_r1_init(c->R1, c->f, t1, d, s);
   if (time_it) HIPTRY(hipEventRecord(c->ev[1], s));
   if (lowp) {
     TRY(lowprec_w2(c, prec));
   } else {
-    TRY(run_gemm(c, G64(c->tau, pv, 1, c->ovvv, 1, pv, c->W2, ov, 1, po, ov, pv, 1.0, 1.0)));
+    TRY(run_gemm(c, G64(c->tau, pv, 1, c->ovvv, 1, pv, c->W2, ov, 1, po, ov, pv, 1.0, 0.0)));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[2], s));
+  k_r1_from_w2(c->R1, c->W2, d, s);
   {
     GemmDesc<double> g = G64(t1, V, 1, c->G, ov, 1, c->U, O * ov, 1, O, V, V, 1.0, 0.0);
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = m, b2 = j
@@ -343,7 +347,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     g.sC1 = V; g.sC2 = ov;
     TRY(mm(g, t1f, lowp ? L->G_f : nullptr));
   }
-  k_w2_finish(c->W2, c->U, d, s);
+  k_w2_finish(c->W2, c->U, c->ooov, d, s);
   if (lowp) k_to_f32(L->W2_f, c->W2, po * ov, s);
 
   // -- W_mbej stored transposed: WT[(j,b),(m,e)]
@@ -463,20 +467,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     }
   }
 
-  // -- T1 right-hand side
-  k_r1_init(c->R1, c->f, t1, d, s);
-  {
-    // -1/2 tau_im^ef <ma||ef>, summed over m
-    GemmDesc<double> g = G64(c->tauu, O * pv, 1, c->ovvv, 1, pv, c->R1, V, 1, O, V, pv, -1.0, 1.0);
-    g.nb1 = (int)O; g.sA1 = pv; g.sB1 = V * pv; g.reduce_batch = true;
-    if (lowp) {
-      GemmDesc<float> f = f32_desc(g, L->tauu_f, L->ovvv32);
-      f.sBn = L->ld_pv; f.sB1 = V * L->ld_pv;
-      TRY(run_f32(c, f));
-    } else {
-      TRY(run_gemm(c, g));
-    }
-  }
+  // -- T1 right-hand side (R1 holds its f and tau.<ma||ef> terms already)
   TRY(mm(G64(c->W3, po * V, 1, c->Tv, V, 1, c->R1, V, 1, O, V, po * V, 1.0, 1.0), lowp ? L->W3_f : nullptr,
          lowp ? L->Tv_f : nullptr));                                                                   // -1/2 t_mn^ae W_mnie
   if (lowp) TRY(lowprec_r1_ring(c, prec));                                                       // t_im^ae F_me

@@ -146,7 +146,7 @@ __global__ void unpack_occ_kernel(T* T2u, T* tauu, const double* t2, const doubl
     double v1 = 0.0, v2 = 0.0;
     if (s != 0.0) { v1 = s * t2[k * d.pv + ab]; v2 = s * tau[k * d.pv + ab]; }
     T2u[idx] = (T)v1;
-    tauu[idx] = (T)v2;
+    if (tauu) tauu[idx] = (T)v2;
   }
 }
 
@@ -261,13 +261,29 @@ __global__ void w3_unpack_kernel(double* W3u, const double* W3, Dims d) {
 }
 
 // W2[ij][(m,b)] += U[i][j][(m,b)] - U[j][i][(m,b)]
-__global__ void w2_finish_kernel(double* W2, const double* U, Dims d) {
+__global__ void w2_finish_kernel(double* W2, const double* U, const double* ooov, Dims d) {
   const long n = d.po * d.ov;
   GRID_LOOP(idx, n) {
     long ij = idx / d.ov, mb = idx % d.ov;
     int i, j;
     pair_decode(ij, i, j);
-    W2[idx] += U[((long)i * d.O + j) * d.ov + mb] - U[((long)j * d.O + i) * d.ov + mb];
+    W2[idx] += ooov[idx] + U[((long)i * d.O + j) * d.ov + mb] - U[((long)j * d.O + i) * d.ov + mb];
+  }
+}
+
+// T1 term -1/2 sum_m sum_ef tau_im^ef <ma||ef> read off the W_mbij product
+// P[ij][(m,b)] = sum_{e<f} tau_ij^ef <mb||ef> (i < j): over all ordered (i,m)
+// the term is -sum_m P(i,m)[(m,a)] with P(m,i) = -P(i,m) and P(i,i) = 0.
+__global__ void r1_from_w2_kernel(double* R1, const double* P, Dims d) {
+  GRID_LOOP(idx, d.ov) {
+    const int i = (int)(idx / d.V), a = (int)(idx % d.V);
+    double s = 0.0;
+    for (int m = 0; m < d.O; ++m) {
+      long k;
+      const double sg = sgn_pair(i, m, k);
+      if (sg != 0.0) s += sg * P[k * d.ov + (long)m * d.V + a];
+    }
+    R1[idx] -= s;
   }
 }
 
@@ -703,7 +719,10 @@ void k_w3_init(double* W3, const double* ooov, Dims d, hipStream_t s) {
 void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s) {
   LAUNCH(w3_unpack_kernel, (long)d.O * d.O * d.O * d.V, W3u, W3, d);
 }
-void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s) { LAUNCH(w2_finish_kernel, d.po * d.ov, W2, U, d); }
+void k_w2_finish(double* W2, const double* U, const double* ooov, Dims d, hipStream_t s) {
+  LAUNCH(w2_finish_kernel, d.po * d.ov, W2, U, ooov, d);
+}
+void k_r1_from_w2(double* R1, const double* P, Dims d, hipStream_t s) { LAUNCH(r1_from_w2_kernel, d.ov, R1, P, d); }
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s) {
   dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
   hipLaunchKernelGGL((wt_from_s_kernel<double, double>), grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s,

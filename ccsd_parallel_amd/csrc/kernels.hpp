@@ -66,7 +66,10 @@ void k_fme_init(double* Fme, const double* f, Dims d, hipStream_t s);
 void k_w1_finish(double* W1, const double* Q, Dims d, hipStream_t s);
 void k_w3_init(double* W3, const double* ooov, Dims d, hipStream_t s);
 void k_w3_unpack(double* W3u, const double* W3, Dims d, hipStream_t s);
-void k_w2_finish(double* W2, const double* U, Dims d, hipStream_t s);
+// W2 += <mb||ij> + P(ij) U (W2 holds the tau.<mb||ef> product on entry)
+void k_w2_finish(double* W2, const double* U, const double* ooov, Dims d, hipStream_t s);
+// R1 -= sum_m P(i,m)[(m,a)] from the packed W_mbij product P (before k_w2_finish)
+void k_r1_from_w2(double* R1, const double* P, Dims d, hipStream_t s);
 void k_fae_t1_extract(double* Fae_t1, const double* WT, Dims d, hipStream_t s);
 // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f  ->  WT[(j,b),(m,e)] = S + <mb||ej>;  F_ae t1 part from S
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s);

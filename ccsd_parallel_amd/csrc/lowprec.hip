@@ -541,9 +541,9 @@ size_t lowprec_plan_bytes(const Dims& d, int precision) {
     const long lpo = pad64(d.po), lv = pad64(V);
     bytes += (size_t)(d.po * lpv + d.po * lpo + d.pv * lpo + d.po * V * lv + V * lv + d.ov * V * lv + O * lv) * 4;
   }
-  // secondary FP32 copies: oovv, oovv_vu, oovv_ou, G, ovvv_u; t1, T2u, tauu, Tv, W1, W3, W3u, W2, Fae, Fmi
+  // secondary FP32 copies: oovv, oovv_vu, oovv_ou, G, ovvv_u; t1, T2u, Tv, W1, W3, W3u, W2, Fae, Fmi
   long sec = d.po * d.pv + d.po * V * V + O * O * d.pv + d.ov * d.ov + d.ov * V * V;
-  sec += d.ov + 2 * O * O * d.pv + d.po * V * V + d.po * d.po + 2 * O * d.po * V + O * O * O * V + d.po * d.ov +
+  sec += d.ov + O * O * d.pv + d.po * V * V + d.po * d.po + 2 * O * d.po * V + O * O * O * V + d.po * d.ov +
          V * V + O * O;
   return bytes + (size_t)sec * 4;
 }
@@ -609,7 +609,6 @@ int lowprec_allocate(Ctx* c, int precision) {
     TRY(lp_alloc(c, &L->ovvv_u_f, d.ov * V * V));
     TRY(lp_alloc(c, &L->t1_f, d.ov));
     TRY(lp_alloc(c, &L->T2u_f, O * O * d.pv));
-    TRY(lp_alloc(c, &L->tauu_f, O * O * d.pv));
     TRY(lp_alloc(c, &L->Tv_f, d.po * V * V));
     TRY(lp_alloc(c, &L->W1_f, d.po * d.po));
     TRY(lp_alloc(c, &L->W3_f, O * d.po * V));
@@ -753,7 +752,7 @@ int lowprec_w2(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
   return nt_product(c, prec, L->tau32, L->tau_b, L->ld_pv, L->ovvv32, L->ovvv_b, L->ld_pv, (int)d.po, (int)d.ov,
-                    (int)d.pv, c->W2, d.ov, 1.0, 1.0);
+                    (int)d.pv, c->W2, d.ov, 1.0, 0.0);
 }
 
 int lowprec_wmbej(Ctx* c, int prec) {

@@ -37,7 +37,7 @@ struct LowPrec {
   // static integrals ...
   float *oovv_f = nullptr, *oovv_vu_f = nullptr, *oovv_ou_f = nullptr, *G_f = nullptr, *ovvv_u_f = nullptr;
   // ... and per-iteration amplitudes / intermediates
-  float *t1_f = nullptr, *T2u_f = nullptr, *tauu_f = nullptr, *Tv_f = nullptr;
+  float *t1_f = nullptr, *T2u_f = nullptr, *Tv_f = nullptr;
   float *W1_f = nullptr, *W3_f = nullptr, *W3u_f = nullptr, *W2_f = nullptr, *Fae_f = nullptr, *Fmi_f = nullptr;
   // BF16 operands of the low-precision Lambda iterations: W_mbej^T and
   // <ia||bc>^T (fixed during a backward) and the per-iteration W_mbej and
