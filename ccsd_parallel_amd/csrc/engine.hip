@@ -350,7 +350,11 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   k_w2_finish(c->W2, c->U, c->ooov, d, s);
   if (lowp) k_to_f32(L->W2_f, c->W2, po * ov, s);
 
-  // -- W_mbej stored transposed: WT[(j,b),(m,e)]
+  // -- W_mbej stored transposed: WT[(j,b),(m,e)].  In the low-precision modes
+  // it accumulates in FP32 instead (lowprec_wmbej_acc: leading dimension ldw),
+  // since its only consumer there is the ring GEMM's FP32 / BF16 operand.
+  long ldw = ov;
+  float* Wf = lowp ? lowprec_wmbej_acc(c, prec, &ldw) : nullptr;
   {
     // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f as one tall GEMM (M = O V^2), staged in X
     // (free until the ring GEMM); then WT = S^(permuted) + <mb||ej>.
@@ -367,7 +371,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
         TRY(run_f32(c, f));
       }
       k_fae_t1_from_s_f32(c->Fae_t1, S32, d, s);
-      k_wt_from_s_f32(c->WT, S32, L->G_f, d, s);
+      k_wt_from_s_f32(Wf, ldw, S32, L->G_f, d, s);
     } else {
       TRY(run_gemm(c, g));
       k_fae_t1_from_s(c->Fae_t1, S, d, s);  // F_ae t1 part = -sum_m S[(m,a,e)][m]
@@ -380,7 +384,15 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     g.nb1 = (int)O; g.nb2 = (int)O;  // b1 = j, b2 = m
     g.sB1 = O * O * V; g.sB2 = O * V;
     g.sC1 = V * ov; g.sC2 = V;
-    TRY(mm(g, t1f, lowp ? L->W3u_f : nullptr));
+    if (lowp) {
+      GemmDesc<float> f = f32_desc(g, t1f, L->W3u_f);
+      f.Cd = nullptr;
+      f.C = Wf;
+      f.sCm = ldw; f.sC1 = V * ldw;
+      TRY(run_f32(c, f));
+    } else {
+      TRY(run_gemm(c, g));
+    }
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[3], s));
   if (lowp) {

@@ -290,8 +290,8 @@ __global__ void r1_from_w2_kernel(double* R1, const double* P, Dims d) {
 // WT[(j,b),(m,e)] = S[(m,b,e)][j] + G[(j,b),(m,e)], S = sum_f <mb||ef> t_j^f.
 // One workgroup per (m,b) and 64-wide chunk of e: the [e][j] block is staged in
 // LDS so that both the read of S and the write of WT are coalesced.
-template <typename TS, typename TG>
-__global__ void wt_from_s_kernel(double* WT, const TS* S, const TG* G, Dims d) {
+template <typename TS, typename TG, typename TO>
+__global__ void wt_from_s_kernel(TO* WT, long ldw, const TS* S, const TG* G, Dims d) {
   extern __shared__ double tile[];  // [64][O+1]
   const int O = d.O, V = d.V;
   const long mb = blockIdx.x;
@@ -307,8 +307,8 @@ __global__ void wt_from_s_kernel(double* WT, const TS* S, const TG* G, Dims d) {
   for (int idx = threadIdx.x; idx < O * 64; idx += blockDim.x) {
     const int j = idx / 64, el = idx % 64;
     if (el >= ne) continue;
-    const long off = ((long)j * V + b) * d.ov + (long)m * V + e0 + el;
-    WT[off] = tile[el * (O + 1) + j] + (double)G[off];
+    const long row = (long)j * V + b, col = (long)m * V + e0 + el;
+    WT[row * ldw + col] = (TO)(tile[el * (O + 1) + j] + (double)G[row * d.ov + col]);
   }
 }
 
@@ -725,13 +725,13 @@ void k_w2_finish(double* W2, const double* U, const double* ooov, Dims d, hipStr
 void k_r1_from_w2(double* R1, const double* P, Dims d, hipStream_t s) { LAUNCH(r1_from_w2_kernel, d.ov, R1, P, d); }
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s) {
   dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
-  hipLaunchKernelGGL((wt_from_s_kernel<double, double>), grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s,
-                     WT, S, G, d);
+  hipLaunchKernelGGL((wt_from_s_kernel<double, double, double>), grid, dim3(kThreads),
+                     64 * (d.O + 1) * sizeof(double), s, WT, d.ov, S, G, d);
 }
-void k_wt_from_s_f32(double* WT, const float* S, const float* G, Dims d, hipStream_t s) {
+void k_wt_from_s_f32(float* WT, long ldw, const float* S, const float* G, Dims d, hipStream_t s) {
   dim3 grid((unsigned)d.ov, (d.V + 63) / 64);
-  hipLaunchKernelGGL((wt_from_s_kernel<float, float>), grid, dim3(kThreads), 64 * (d.O + 1) * sizeof(double), s,
-                     WT, S, G, d);
+  hipLaunchKernelGGL((wt_from_s_kernel<float, float, float>), grid, dim3(kThreads),
+                     64 * (d.O + 1) * sizeof(double), s, WT, ldw, S, G, d);
 }
 void k_fae_t1_from_s(double* F, const double* S, Dims d, hipStream_t s) {
   LAUNCH(fae_t1_from_s_kernel<double>, (long)d.V * d.V, F, S, d);

@@ -73,7 +73,7 @@ void k_r1_from_w2(double* R1, const double* P, Dims d, hipStream_t s);
 void k_fae_t1_extract(double* Fae_t1, const double* WT, Dims d, hipStream_t s);
 // S[(m,b,e)][j] = sum_f <mb||ef> t_j^f  ->  WT[(j,b),(m,e)] = S + <mb||ej>;  F_ae t1 part from S
 void k_wt_from_s(double* WT, const double* S, const double* G, Dims d, hipStream_t s);
-void k_wt_from_s_f32(double* WT, const float* S, const float* G, Dims d, hipStream_t s);
+void k_wt_from_s_f32(float* WT, long ldw, const float* S, const float* G, Dims d, hipStream_t s);
 void k_fae_t1_from_s(double* Fae_t1, const double* S, Dims d, hipStream_t s);
 void k_fae_t1_from_s_f32(double* F, const float* S, Dims d, hipStream_t s);
 void k_fmi_init(double* Fmi, const double* f, const double* ooov, const double* t1, Dims d, hipStream_t s);

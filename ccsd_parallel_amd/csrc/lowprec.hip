@@ -136,9 +136,14 @@ __global__ void gemv_lp_kernel(double* y, const float* A32, const uint16_t* Ab, 
 // the FP64 C64[row*ldc + col] = alpha * acc + beta * C64, optionally also
 // written as interleaved hi/lo rows Ob (leading dimension ldo) for use as a
 // later operand.
+// With C32 set instead, the FP32 C32 is the accumulator: v = alpha * acc +
+// beta * C32, stored back to C32 unless c32_keep is false (when only the hi/lo
+// operand Ob is wanted).
 struct Bf16Out {
   float* C = nullptr;
   double* C64 = nullptr;
+  float* C32 = nullptr;
+  bool c32_keep = true;
   long ldc = 0;
   double alpha = 1.0, beta = 0.0;
   uint16_t* Ob = nullptr;
@@ -176,11 +181,16 @@ __device__ __forceinline__ void bf16_tile_coords(const Bf16Args& p, int tile, in
 }
 
 __device__ __forceinline__ void bf16_store(const Bf16Out& o, int row, int col, float acc) {
-  if (o.C64) {
-    double* c = o.C64 + (long)row * o.ldc + col;
+  if (o.C64 || o.C32) {
+    const long off = (long)row * o.ldc + col;
     double v = o.alpha * (double)acc;
-    if (o.beta != 0.0) v += o.beta * (*c);
-    *c = v;
+    if (o.C64) {
+      if (o.beta != 0.0) v += o.beta * o.C64[off];
+      o.C64[off] = v;
+    } else {
+      if (o.beta != 0.0) v += o.beta * (double)o.C32[off];
+      if (o.c32_keep) o.C32[off] = (float)v;
+    }
     if (o.Ob) {
       uint16_t h, l;
       bf16_split(v, h, l);
@@ -630,8 +640,9 @@ int lowprec_allocate(Ctx* c, int precision) {
     TRY(lp_alloc(c, &L->Doovv32, d.ov * L->ld_ov));
     TRY(lp_alloc(c, &L->Tring32, d.ov * L->ld_ov));
     TRY(lp_alloc(c, &L->WT32, d.ov * L->ld_ov));
-    // the unpack writes Tring32 for columns < ov only: zero the padding once
-    HIPTRY(hipMemsetAsync(L->Tring32, 0, (size_t)d.ov * L->ld_ov * 4, s));
+    // the unpack writes Tring32, the W_mbej build WT32, for columns < ov only:
+    // zero the padding once
+    for (float* q : {L->Tring32, L->WT32}) HIPTRY(hipMemsetAsync(q, 0, (size_t)d.ov * L->ld_ov * 4, s));
     hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.pv * L->ld_pv)), dim3(kThreads), 0, s, L->vvvv32, c->vvvv,
                        d.pv, d.pv, d.pv, L->ld_pv, 1.0);
     hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, s, L->Doovv32, c->Doovv,
@@ -723,14 +734,16 @@ int lowprec_r1_ring(Ctx* c, int prec) {
 
 // C64 (M x N, ldc) = alpha * A B^T + beta * C64 on the stored low-precision
 // operands; in BF16 mode the product can also be emitted as an interleaved
-// hi/lo operand (Ob, leading dimension ldo).
+// hi/lo operand (Ob, leading dimension ldo).  With C null the FP32 C32 is the
+// destination instead (in BF16 mode stored back only if keep32).
 static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, long lda, const float* B32,
                       const uint16_t* Bb, long ldb, int M, int N, int K, double* C, long ldc, double alpha,
-                      double beta, uint16_t* Ob = nullptr, long ldo = 0) {
+                      double beta, uint16_t* Ob = nullptr, long ldo = 0, float* C32 = nullptr, bool keep32 = true) {
   hipError_t e;
   if (prec == CCSD_PREC_FP32) {
     GemmDesc<float> g;
     g.A = A32; g.B = B32; g.Cd = C;
+    if (!C) g.C = C32;
     g.M = M; g.N = N; g.K = K;
     g.sAm = lda; g.sAk = 1; g.sBk = 1; g.sBn = ldb; g.sCm = ldc; g.sCn = 1;
     g.alpha = (float)alpha; g.beta = (float)beta;  // 1, 0.5, 0: exact in FP32
@@ -738,6 +751,7 @@ static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, lo
   } else {
     Bf16Out o;
     o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
+    if (!C) { o.C32 = C32; o.c32_keep = keep32; }
     o.Ob = Ob; o.ldo = ldo;
     e = bf16x3_nt(Ab, lda, Bb, ldb, o, M, N, K, c->stream, -1, &c->ws);
   }
@@ -755,13 +769,27 @@ int lowprec_w2(Ctx* c, int prec) {
                     (int)d.pv, c->W2, d.ov, 1.0, 0.0);
 }
 
+float* lowprec_wmbej_acc(Ctx* c, int prec, long* ld) {
+  LowPrec* L = c->lp;
+  const Dims& d = c->d;
+  if (prec == CCSD_PREC_FP32) {
+    *ld = L->ld_ov;
+    return L->WT32;
+  }
+  *ld = d.ov;
+  return reinterpret_cast<float*>(c->X) + d.ov * d.ov;
+}
+
 int lowprec_wmbej(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
-  // Doovv is symmetric, so its rows serve as the k-contiguous B operand.  In BF16
-  // mode the finished W_mbej is also written as the ring GEMM's hi/lo operand.
+  long ld;
+  float* W = lowprec_wmbej_acc(c, prec, &ld);
+  // Doovv is symmetric, so its rows serve as the k-contiguous B operand.  FP32:
+  // W is the ring GEMM's operand itself.  BF16: the finished W_mbej is written
+  // only as the ring GEMM's hi/lo operand.
   return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->Doovv32, L->Doovv_b, L->ld_ov, (int)d.ov,
-                    (int)d.ov, (int)d.ov, c->WT, d.ov, 0.5, 1.0, L->WT_b, L->ld_ov);
+                    (int)d.ov, (int)d.ov, nullptr, ld, 0.5, 1.0, L->WT_b, L->ld_ov, W, false);
 }
 
 int lowprec_ladder(Ctx* c, int prec) {
@@ -776,10 +804,7 @@ int lowprec_ladder(Ctx* c, int prec) {
 int lowprec_ring(Ctx* c, int prec) {
   LowPrec* L = c->lp;
   const Dims& d = c->d;
-  // BF16: WT_b comes from the W_mbej epilogue (WT is final there)
-  if (prec == CCSD_PREC_FP32)
-    hipLaunchKernelGGL(conv_f32_kernel, dim3(grid_for(d.ov * L->ld_ov)), dim3(kThreads), 0, c->stream, L->WT32,
-                       c->WT, d.ov, d.ov, d.ov, L->ld_ov, 1.0);
+  // the W_mbej operand (WT32 / WT_b) is final after lowprec_wmbej
   return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->WT32, L->WT_b, L->ld_ov, (int)d.ov, (int)d.ov,
                     (int)d.ov, c->X, d.ov, 1.0, 0.0);
 }

@@ -65,6 +65,11 @@ RingOut lowprec_ring_out(Ctx* c, int precision);
 int lowprec_fme(Ctx* c, int precision, const double* t1);
 int lowprec_r1_ring(Ctx* c, int precision);
 int lowprec_w2(Ctx* c, int precision);
+// FP32 accumulator of W_mbej (stored [(j,b),(m,e)], leading dimension *ld) in
+// the low-precision modes: the FP32 ring operand itself, or in BF16 mode the
+// upper half of X (free until the ring GEMM writes X).
+float* lowprec_wmbej_acc(Ctx* c, int precision, long* ld);
+// W_mbej += 1/2 Tring Doovv on the accumulator; BF16 mode emits the ring operand
 int lowprec_wmbej(Ctx* c, int precision);
 int lowprec_ladder(Ctx* c, int precision);
 int lowprec_ring(Ctx* c, int precision);
