@@ -509,10 +509,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
 int device_energy(Ctx* c, const double* t1, const double* t2, double* e) {
   const Dims& d = c->d;
   hipStream_t s = c->stream;
-  k_build_tau(c->tau, t2, t1, d, s);
-  const long n2 = d.po * d.pv;
-  const int nb2 = reduce_blocks_for(n2), nb1 = reduce_blocks_for(d.ov);
-  k_dot_partials(c->partial, c->oovv, c->tau, n2, nb2, s);
+  const int nb2 = k_energy_partials(c->partial, c->oovv, t2, t1, d, s), nb1 = reduce_blocks_for(d.ov);
   k_dot_partials(c->partial + nb2, c->fov, t1, d.ov, nb1, s);
   k_sum_partials(c->scal, c->partial, nb2, 1, s);
   k_sum_partials(c->scal + 1, c->partial + nb2, nb1, 1, s);

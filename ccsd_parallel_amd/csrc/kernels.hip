@@ -510,6 +510,27 @@ __global__ void dot_partials_kernel(double* partial, const double* x, const doub
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Energy partials sum_{i<j, a<b} <ij||ab> tau_ij^ab with tau formed on the fly
+// (t2 + t_i^a t_j^b - t_i^b t_j^a): threads over ab, blockIdx.y over ij.
+__global__ void energy_partials_kernel(double* partial, const double* oovv, const double* t2, const double* t1,
+                                       Dims d) {
+  double s = 0.0;
+  for (long ab = blockIdx.x * (long)blockDim.x + threadIdx.x; ab < d.pv; ab += (long)gridDim.x * blockDim.x) {
+    int a, b;
+    pair_decode(ab, a, b);
+    for (long ij = blockIdx.y; ij < d.po; ij += gridDim.y) {
+      int i, j;
+      pair_decode(ij, i, j);
+      const long e = ij * d.pv + ab;
+      const double tau =
+          t2[e] + t1[(long)i * d.V + a] * t1[(long)j * d.V + b] - t1[(long)i * d.V + b] * t1[(long)j * d.V + a];
+      s += oovv[e] * tau;
+    }
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
 __global__ void maxabs_partials_kernel(double* partial, const double* x, long n) {
   double s = 0.0;
   GRID_LOOP(i, n) s = fmax(s, fabs(x[i]));
@@ -772,6 +793,12 @@ void k_update_t1(double* t1n, double* err, const double* R1, const double* f, co
 
 int reduce_blocks_for(long n) { return (int)std::max<long>(1, std::min<long>((n + kThreads - 1) / kThreads, 1024)); }
 
+int k_energy_partials(double* partial, const double* oovv, const double* t2, const double* t1, Dims d, hipStream_t s) {
+  const unsigned gx = (unsigned)std::min<long>((d.pv + kThreads - 1) / kThreads, 256);
+  const unsigned gy = (unsigned)std::min<long>(std::max<long>(d.po, 1), 64);
+  hipLaunchKernelGGL(energy_partials_kernel, dim3(gx, gy), dim3(kThreads), 0, s, partial, oovv, t2, t1, d);
+  return (int)(gx * gy);
+}
 void k_dot_partials(double* partial, const double* x, const double* y, long n, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(dot_partials_kernel, dim3(nblocks), dim3(kThreads), 0, s, partial, x, y, n);
 }

@@ -96,6 +96,9 @@ void k_update_t1(double* t1n, double* err, const double* R1, const double* f, co
 // partial[b] = sum over block b of x.y ; host then reduces num_blocks values.
 int reduce_blocks_for(long n);
 void k_dot_partials(double* partial, const double* x, const double* y, long n, int nblocks, hipStream_t s);
+// Partials of sum_{i<j,a<b} <ij||ab> (t2 + P t1 t1) (tau not stored); returns
+// the number of partials written (at most 16384)
+int k_energy_partials(double* partial, const double* oovv, const double* t2, const double* t1, Dims d, hipStream_t s);
 void k_maxabs_partials(double* partial, const double* x, long n, int nblocks, hipStream_t s);
 // partial[v*nblocks + b] for nv vectors x_v (stride ldx) against y
 void k_multidot_partials(double* partial, const double* X, long ldx, int nv, const double* y, long n, int nblocks,
