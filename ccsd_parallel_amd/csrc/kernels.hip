@@ -135,18 +135,24 @@ __global__ void build_tau_kernel(double* tau, const double* t2, const double* t1
   }
 }
 
+// T2u[i][m][ab] = t_im^ab (antisymmetric in i,m; zero for i = m), tauu likewise
+// from tau.  Block row y = (i, m), threads over ab: the packed source row and
+// its sign are fixed per block, so no per-element index arithmetic remains.
 template <typename T>
 __global__ void unpack_occ_kernel(T* T2u, T* tauu, const double* t2, const double* tau, Dims d) {
-  const long n = (long)d.O * d.O * d.pv;
-  GRID_LOOP(idx, n) {
-    long im = idx / d.pv, ab = idx % d.pv;
-    int i = (int)(im / d.O), m = (int)(im % d.O);
+  const long nim = (long)d.O * d.O;
+  for (long im = blockIdx.y; im < nim; im += gridDim.y) {
+    const int i = (int)(im / d.O), m = (int)(im % d.O);
     long k;
-    double s = sgn_pair(i, m, k);
-    double v1 = 0.0, v2 = 0.0;
-    if (s != 0.0) { v1 = s * t2[k * d.pv + ab]; v2 = s * tau[k * d.pv + ab]; }
-    T2u[idx] = (T)v1;
-    if (tauu) tauu[idx] = (T)v2;
+    const double sg = sgn_pair(i, m, k);
+    const double* s1 = t2 + k * d.pv;
+    const double* s2 = tau + k * d.pv;
+    T* o1 = T2u + im * d.pv;
+    T* o2 = tauu ? tauu + im * d.pv : nullptr;
+    for (long ab = blockIdx.x * (long)blockDim.x + threadIdx.x; ab < d.pv; ab += (long)gridDim.x * blockDim.x) {
+      o1[ab] = (T)(sg != 0.0 ? sg * s1[ab] : 0.0);
+      if (o2) o2[ab] = (T)(sg != 0.0 ? sg * s2[ab] : 0.0);
+    }
   }
 }
 
@@ -710,11 +716,15 @@ void k_build_tau(double* tau, const double* t2, const double* t1, Dims d, hipStr
   const dim3 grid((unsigned)((d.pv + kThreads - 1) / kThreads), (unsigned)std::min<long>(d.po, 65535));
   hipLaunchKernelGGL(build_tau_kernel, grid, dim3(kThreads), 0, s, tau, t2, t1, d);
 }
+static dim3 occ_pair_grid(const Dims& d) {
+  return dim3((unsigned)std::min<long>((d.pv + kThreads - 1) / kThreads, 64),
+              (unsigned)std::min<long>(std::max<long>((long)d.O * d.O, 1), 65535));
+}
 void k_unpack_occ(double* T2u, double* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
-  LAUNCH(unpack_occ_kernel<double>, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
+  hipLaunchKernelGGL(unpack_occ_kernel<double>, occ_pair_grid(d), dim3(kThreads), 0, s, T2u, tauu, t2, tau, d);
 }
 void k_unpack_occ_f32(float* T2u, float* tauu, const double* t2, const double* tau, Dims d, hipStream_t s) {
-  LAUNCH(unpack_occ_kernel<float>, (long)d.O * d.O * d.pv, T2u, tauu, t2, tau, d);
+  hipLaunchKernelGGL(unpack_occ_kernel<float>, occ_pair_grid(d), dim3(kThreads), 0, s, T2u, tauu, t2, tau, d);
 }
 void k_unpack_vir_ring(double* Tv, double* Tr, const double* t2, Dims d, hipStream_t s) {
   RingOut r;

@@ -43,19 +43,32 @@ __device__ __forceinline__ long il_index(long row, long k, long ld) { return row
 
 // dst (interleaved, rows x ld) = split(scale * src[r*sld + k]) for k < K, 0 beyond;
 // one thread per 8-element chunk (two 16-byte stores).
+// Whole chunks of a 16-byte aligned source with a row stride that keeps the
+// alignment are read with 16-byte loads.
 template <typename S>
 __global__ void conv_bf16il_kernel(uint16_t* dst, const S* src, long rows, long K, long sld, long ld,
                                    double scale) {
   const long chunks = ld / 8, n = rows * chunks;
+  constexpr int kPer16 = 16 / (int)sizeof(S);
+  const bool vec = (reinterpret_cast<uintptr_t>(src) % 16 == 0) && (sld % kPer16 == 0);
   GRID_LOOP(idx, n) {
     const long r = idx / chunks, c = idx % chunks;
     uint16_t h[8], l[8];
+    if (vec && c * 8 + 8 <= K) {
+      S v[8];
+      const uint4* p = reinterpret_cast<const uint4*>(src + r * sld + c * 8);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const long k = c * 8 + e;
-      h[e] = 0;
-      l[e] = 0;
-      if (k < K) bf16_split(scale * (double)src[r * sld + k], h[e], l[e]);
+      for (int q = 0; q < 8 / kPer16; ++q) *reinterpret_cast<uint4*>(v + q * kPer16) = p[q];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bf16_split(scale * (double)v[e], h[e], l[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const long k = c * 8 + e;
+        h[e] = 0;
+        l[e] = 0;
+        if (k < K) bf16_split(scale * (double)src[r * sld + k], h[e], l[e]);
+      }
     }
     uint4 hv, lv;
     hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
