@@ -604,17 +604,13 @@ int rccsd_rhs(X& x, long o, long v, const RStatic& S, const RWork& W, double* t1
   RT(x.xk_unpack(W.XS, W.XA, W.Xk, o, v));
   RT(x.ein("kiac,jc->kija", 1.0, ovovK, t1, 1.0, Xk));
   RT(x.axpy("iakj->kija", 1.0, ovoo, 1.0, Xk));
-  RT(x.ein("kija,kb->ijab", 1.0, Xk, t1, 0.0, Y));
-  RT(x.axpy("ijab->ijab", -1.0, Y, 1.0, R2));
-  RT(x.axpy("jiba->ijab", -1.0, Y, 1.0, R2));
-  // (ai|bc) t_jc - t_ka (ki|bc) t_jc
-  RT(x.ein("iabc,jc->ijab", 1.0, ovvv, t1, 0.0, Y));
+  // The P-symmetrised terms accumulate in Y, then R2 += Y + P Y once:
+  // -Xk t1; (ai|bc) t_jc - t_ka (ki|bc) t_jc; Lvv, Loo
+  RT(x.ein("kija,kb->ijab", -1.0, Xk, t1, 0.0, Y));
+  RT(x.ein("iabc,jc->ijab", 1.0, ovvv, t1, 1.0, Y));
   RT(x.ein("kibc,jc->kibj", 1.0, oovv, t1, 0.0, H));
   RT(x.ein("ka,kibj->ijab", -1.0, t1, H, 1.0, Y));
-  RT(x.axpy("ijab->ijab", 1.0, Y, 1.0, R2));
-  RT(x.axpy("jiba->ijab", 1.0, Y, 1.0, R2));
-  // Lvv, Loo
-  RT(x.ein("ijac,bc->ijab", 1.0, t2, Lvv, 0.0, Y));
+  RT(x.ein("ijac,bc->ijab", 1.0, t2, Lvv, 1.0, Y));
   RT(x.ein("ki,kjab->ijab", -1.0, Loo, t2, 1.0, Y));
   RT(x.axpy("ijab->ijab", 1.0, Y, 1.0, R2));
   RT(x.axpy("jiba->ijab", 1.0, Y, 1.0, R2));
