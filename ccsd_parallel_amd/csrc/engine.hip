@@ -36,6 +36,28 @@ namespace {
 
 }  // namespace
 
+// K splits of the FP64 pp ladder: whole-K tiles re-stream the tau panel set
+// (P2(O) x P2(V) doubles, 280 MB at O=40/V=300) once per column tile on each
+// XCD, and a set larger than half the 256 MB last-level cache comes from HBM
+// each time.  The ladder is split into the fewest k-chunks whose tau panels fit
+// (measured at O=40/V=300: 3 chunks 47.1 ms against 48.1 ms for whole-K tiles
+// with the split tail; 2 chunks 49.6 ms); 0 (modelled choice) when the whole
+// set fits.
+static int ladder_splits(const Dims& d) {
+  const double set = (double)d.po * (double)d.pv * sizeof(double), half_llc = 128e6;
+  if (set <= half_llc) return 0;
+  return (int)std::ceil(set / half_llc);
+}
+
+// GEMM workspace of a context: room for the k-split partial slices of the
+// packed pp ladder (P2(O) x P2(V) each) and for the W_mbij splits, at least
+// kGemmWorkspaceBytes.
+static size_t workspace_bytes(const Dims& d) {
+  const size_t slices = (size_t)std::max(3, ladder_splits(d));
+  const size_t ladder = slices * (size_t)d.po * (size_t)d.pv * sizeof(double) + (16ull << 20);
+  return std::max(kGemmWorkspaceBytes, (ladder + 255) / 256 * 256);
+}
+
 size_t plan_bytes(int O, int V, int precision, int diis_m) {
   Dims d = make_dims(O, V);
   const long N = O + V;
@@ -56,7 +78,7 @@ size_t plan_bytes(int O, int V, int precision, int diis_m) {
   // temporary ovov at setup shares nothing: count it
   n += d.ov * d.ov;
   size_t bytes = (size_t)n * sizeof(double);
-  bytes += kGemmWorkspaceBytes;  // GEMM workspace
+  bytes += workspace_bytes(d);  // GEMM workspace
   bytes += 1ull << 20;   // reductions
   bytes += lowprec_plan_bytes(d, precision);
   return bytes;
@@ -137,9 +159,10 @@ int plan_and_allocate(Ctx* c, int precision) {
   TRY(dalloc(c, &c->partial, 1 << 16));
   TRY(dalloc(c, &c->scal, 64));
   double* ws = nullptr;
-  TRY(dalloc(c, &ws, (long)(kGemmWorkspaceBytes / 8)));
+  const size_t wsb = workspace_bytes(d);
+  TRY(dalloc(c, &ws, (long)(wsb / 8)));
   c->ws.ptr = ws;
-  c->ws.bytes = kGemmWorkspaceBytes;
+  c->ws.bytes = wsb;
   // Low-precision operand copies are made on first use (lowprec_amplitudes),
   // after the integrals are in HBM; plan_bytes above already reserved them.
   for (auto& e : c->ev) HIPTRY(hipEventCreate(&e));
@@ -424,7 +447,8 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (lowp) {
     TRY(lowprec_ladder(c, prec));
   } else {
-    TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, 1, pv, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0)));
+    TRY(run_gemm(c, G64(c->tau, pv, 1, c->vvvv, 1, pv, c->Rp, pv, 1, po, pv, pv, 1.0, 1.0), nullptr,
+                 ladder_splits(d)));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[6], s));
   {

@@ -96,8 +96,8 @@ inline GemmDesc<double> G64(const double* A, long sAm, long sAk, const double* B
 
 // GEMM helper with error propagation
 template <typename T>
-inline int run_gemm(Ctx* c, const GemmDesc<T>& g, const GemmWorkspace* ws = nullptr) {
-  hipError_t e = gemm<T>(g, ws ? *ws : c->ws, c->stream);
+inline int run_gemm(Ctx* c, const GemmDesc<T>& g, const GemmWorkspace* ws = nullptr, int force_split = 0) {
+  hipError_t e = gemm<T>(g, ws ? *ws : c->ws, c->stream, nullptr, -1, force_split);
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;
