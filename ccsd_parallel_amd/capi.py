@@ -365,11 +365,12 @@ def strerror(code):
 
 
 def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision="fp64", tile=0,
-                  elementwise=False):
+                  elementwise=False, strassen=False):
     """C = alpha op(A) op(B) + beta C through the device GEMM engine; tile > 0
     forces that tile configuration (index + 1) instead of the automatic choice;
     elementwise forces single-element global loads where the FP64 engine would
-    load element pairs."""
+    load element pairs; strassen runs one level of Strassen's method (FP64,
+    square, even size, beta 0)."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     Cm = np.ascontiguousarray(C_in, dtype=np.float64).copy()
@@ -378,7 +379,7 @@ def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision
     N = B.shape[0] if tb else B.shape[1]
     _check(load().ccsd_selftest_gemm(M, N, K, int(ta), int(tb), alpha, _ptr(A), _ptr(B), beta, _ptr(Cm),
                                      PRECISIONS.get(precision, precision) | (int(tile) << 8)
-                                     | (int(bool(elementwise)) << 16)),
+                                     | (int(bool(elementwise)) << 16) | (int(bool(strassen)) << 17)),
            "ccsd_selftest_gemm")
     return Cm
 

@@ -17,6 +17,7 @@
 
 #include "adjoint.hpp"
 #include "engine.hpp"
+#include "strassen.hpp"
 #include "lowprec.hpp"
 
 struct ccsd_ctx {
@@ -717,8 +718,17 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
       g.alpha = alpha; g.beta = beta;
       const int cfg = ((precision >> 8) & 0xff) - 1;  // forced tile configuration, -1 = automatic
       const int pairs = (precision >> 16) & 1 ? 0 : -1;  // bit 16: element-wise loads only
-      if (rc == CCSD_OK && gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, 0, pairs) != hipSuccess)
+      if ((precision >> 17) & 1) {  // bit 17: one level of Strassen (M = N = K even, beta 0)
+        double* tmp = nullptr;
+        if (rc == CCSD_OK && (!strassen_applies(g, 2) ||
+                              hipMalloc(&tmp, strassen_temp_elems(M) * sizeof(double)) != hipSuccess))
+          rc = CCSD_ERR_ARG;
+        if (rc == CCSD_OK && gemm_strassen(g, tmp, GemmWorkspace{ws, wsb}, s) != hipSuccess) rc = CCSD_ERR_HIP;
+        if (rc == CCSD_OK && hipStreamSynchronize(s) != hipSuccess) rc = CCSD_ERR_HIP;
+        if (tmp) (void)hipFree(tmp);
+      } else if (rc == CCSD_OK && gemm<double>(g, GemmWorkspace{ws, wsb}, s, nullptr, cfg, 0, pairs) != hipSuccess) {
         rc = CCSD_ERR_HIP;
+      }
     } else {
       // low-precision path wants A as [M][K] and B as [N][K]
       std::vector<double> Ak((size_t)M * K), Bk((size_t)N * K);

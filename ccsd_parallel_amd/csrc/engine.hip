@@ -14,6 +14,7 @@
 
 #include "engine.hpp"
 #include "lowprec.hpp"
+#include "strassen.hpp"
 
 namespace ccsd {
 
@@ -79,6 +80,7 @@ size_t plan_bytes(int O, int V, int precision, int diis_m) {
   n += d.ov * d.ov;
   size_t bytes = (size_t)n * sizeof(double);
   bytes += workspace_bytes(d);  // GEMM workspace
+  if (d.ov % 2 == 0 && d.ov >= kStrassenMinN) bytes += strassen_temp_elems(d.ov) * sizeof(double);
   bytes += 1ull << 20;   // reductions
   bytes += lowprec_plan_bytes(d, precision);
   return bytes;
@@ -161,6 +163,7 @@ int plan_and_allocate(Ctx* c, int precision) {
   double* ws = nullptr;
   const size_t wsb = workspace_bytes(d);
   TRY(dalloc(c, &ws, (long)(wsb / 8)));
+  if (d.ov % 2 == 0 && d.ov >= kStrassenMinN) TRY(dalloc(c, &c->strassen_tmp, (long)strassen_temp_elems(d.ov)));
   c->ws.ptr = ws;
   c->ws.bytes = wsb;
   // Low-precision operand copies are made on first use (lowprec_amplitudes),
@@ -275,6 +278,27 @@ static int run_f32(Ctx* c, const GemmDesc<float>& g) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;
   }
+  return CCSD_OK;
+}
+
+// The (OV)^3 FP64 products: one level of Strassen (strassen.hip) when the
+// context holds its temporaries, else the classical engine call.  With
+// beta != 0 the Strassen product goes to `scratch` (OV x OV) first and is
+// added to C after.
+static int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch) {
+  GemmDesc<double> h = g;
+  if (g.beta != 0.0) {
+    h.C = scratch;
+    h.beta = 0.0;
+  }
+  if (!c->strassen_tmp || (g.beta != 0.0 && (!scratch || g.beta != 1.0 || g.sCm != g.N)) || !strassen_applies(h))
+    return run_gemm(c, g);
+  hipError_t e = gemm_strassen(h, c->strassen_tmp, c->ws, c->stream);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
+  if (g.beta != 0.0) k_axpy(g.C, scratch, 1.0, (long)g.M * g.N, c->stream);
   return CCSD_OK;
 }
 
@@ -421,8 +445,9 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (lowp) {
     TRY(lowprec_wmbej(c, prec));
   } else {
-    // + 1/2 t_jn^bf <mn||ef>
-    TRY(run_gemm(c, G64(c->Tring, ov, 1, c->Doovv, ov, 1, c->WT, ov, 1, ov, ov, ov, 0.5, 1.0)));
+    // + 1/2 t_jn^bf <mn||ef>: one-level Strassen into X (free here: S is
+    // consumed), then WT += X; the classical product accumulates directly
+    TRY(square_product(c, G64(c->Tring, ov, 1, c->Doovv, ov, 1, c->WT, ov, 1, ov, ov, ov, 0.5, 1.0), c->X));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[4], s));
 
@@ -469,7 +494,7 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (lowp) {
     TRY(lowprec_ring(c, prec));
   } else {
-    TRY(run_gemm(c, G64(c->Tring, ov, 1, c->WT, 1, ov, c->X, ov, 1, ov, ov, ov, 1.0, 0.0)));
+    TRY(square_product(c, G64(c->Tring, ov, 1, c->WT, 1, ov, c->X, ov, 1, ov, ov, ov, 1.0, 0.0), nullptr));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[8], s));
 

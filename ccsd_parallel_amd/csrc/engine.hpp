@@ -43,6 +43,8 @@ struct Ctx {
   double *Fme = nullptr, *Fmi = nullptr, *Fae = nullptr, *Fae_t1 = nullptr;
   double *W1 = nullptr, *Q = nullptr, *W3 = nullptr, *W3u = nullptr, *W2 = nullptr, *U = nullptr;
   double *WT = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr, *Rp = nullptr, *R1 = nullptr;
+  // temporaries of the one-level Strassen (OV)^3 products (FP64), or null
+  double* strassen_tmp = nullptr;
 
   // DIIS
   int diis_m = 0, diis_count = 0, diis_next = 0;

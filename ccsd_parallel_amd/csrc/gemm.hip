@@ -683,9 +683,6 @@ constexpr Cfg kCfgs[] = {
     // paired loads of an n-contiguous B need more than its 128 registers
     // (spills), so it serves k-contiguous B only (W_mbij: 61.1 TF/s)
     {112, 128, 16, 1, 8, 2, 67.0f, true, true, true},
-    // 8 waves of 112 x 32 (one workgroup per CU: 106 KB of LDS): half the LDS
-    // reads per flop of the 112 x 16 wave tile above
-    {112, 256, 16, 1, 8, 1, 1.0f, true, true, true},
 };
 constexpr int kNumCus = 256;
 constexpr int kTileBand = 8;  // tile rows per band of the tile walk (tile_coords)
@@ -739,11 +736,10 @@ hipError_t launch_any(int c, const KArgs<T>& ka, bool AK, bool BNM, int vec, dim
     case 13: return launch_cfg<T, 13>(ka, AK, BNM, vec, grid, s);
     case 14: return launch_cfg<T, 14>(ka, AK, BNM, vec, grid, s);
     case 15: return launch_cfg<T, 15>(ka, AK, BNM, vec, grid, s);
-    case 16: return launch_cfg<T, 16>(ka, AK, BNM, vec, grid, s);
-    default: return launch_cfg<T, 17>(ka, AK, BNM, vec, grid, s);
+    default: return launch_cfg<T, 16>(ka, AK, BNM, vec, grid, s);
   }
 }
-static_assert(kNumCfgs == 18, "launch_any covers every configuration");
+static_assert(kNumCfgs == 17, "launch_any covers every configuration");
 
 // W-element vector loads are legal when each operand has unit stride along
 // its vector index, every other stride and batch stride is a multiple of W,

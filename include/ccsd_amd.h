@@ -283,7 +283,8 @@ long ccsd_debug_get(ccsd_ctx* c, const char* name, double* out, long cap);
  * BF16X3 the hi/lo kernel's (1 = 128x128, 2 = 256x256 with the automatic
  * tile order, 3 = 256x256 plain order, 4 = 256x256 grouped order).  Bit 16
  * (FP64) forces single-element global loads where the engine would load
- * element pairs. */
+ * element pairs.  Bit 17 (FP64, M = N = K even, beta = 0) runs the product
+ * through one level of Strassen's method (seven half-size engine calls). */
 int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const double* A, const double* B,
                        double beta, double* C, int precision);
 

@@ -54,6 +54,25 @@ def test_gemm_fp64(M, N, K, ta, tb):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("n,ta,tb", [(64, 0, 1), (64, 0, 0), (998, 0, 1), (1000, 1, 0), (1000, 1, 1), (1026, 0, 0)])
+def test_gemm_fp64_strassen(n, ta, tb):
+    """One level of Strassen's method (Winograd's form) through the engine:
+    every orientation of A and B (k- or m/n-contiguous), half sizes with ragged
+    tiles (499, 513), against the classical product.  The bound allows the
+    small constant-factor growth of the error over the classical product."""
+    capi = _ctx()
+    rng = np.random.default_rng(n + 10 * ta + tb)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, n))
+    C0 = rng.standard_normal((n, n))
+    ref = 0.7 * ((A.T if ta else A) @ (B.T if tb else B))
+    out = capi.selftest_gemm(A, B, C0, 0.7, 0.0, ta=bool(ta), tb=bool(tb), precision="fp64", strassen=True)
+    scale = np.abs(A).max() * np.abs(B).max() * n
+    assert np.max(np.abs(out - ref)) <= 1e-14 * scale
+    with pytest.raises(capi.CCSDError):   # beta != 0 is not a Strassen product
+        capi.selftest_gemm(A, B, C0, 0.7, 0.5, precision="fp64", strassen=True)
+
+
 N_FP64_TILES = 17  # configurations of the generic MFMA engine
 N_BF16_TILES = 6   # configurations of the BF16 hi/lo kernel
 
