@@ -43,19 +43,22 @@ namespace {
 // each time.  The ladder is split into the fewest k-chunks whose tau panels fit
 // (measured at O=40/V=300: 3 chunks 47.1 ms against 48.1 ms for whole-K tiles
 // with the split tail; 2 chunks 49.6 ms); 0 (modelled choice) when the whole
-// set fits.
+// set fits, or when more than 4 chunks or 4 GB of partial slices would be
+// needed (not measured there; the workspace stays small).
 static int ladder_splits(const Dims& d) {
   const double set = (double)d.po * (double)d.pv * sizeof(double), half_llc = 128e6;
   if (set <= half_llc) return 0;
-  return (int)std::ceil(set / half_llc);
+  const int s = (int)std::ceil(set / half_llc);
+  return (s > 4 || s * set > 4e9) ? 0 : s;
 }
 
-// GEMM workspace of a context: room for the k-split partial slices of the
-// packed pp ladder (P2(O) x P2(V) each) and for the W_mbij splits, at least
-// kGemmWorkspaceBytes.
+// GEMM workspace of a context: at least kGemmWorkspaceBytes, and room for the
+// pp ladder's k-split partial slices (P2(O) x P2(V) each) when it is split
+// (which also gives W_mbij more k-splits: 3 -> 10 at O=40/V=300).
 static size_t workspace_bytes(const Dims& d) {
-  const size_t slices = (size_t)std::max(3, ladder_splits(d));
-  const size_t ladder = slices * (size_t)d.po * (size_t)d.pv * sizeof(double) + (16ull << 20);
+  const int ls = ladder_splits(d);
+  if (ls == 0) return kGemmWorkspaceBytes;
+  const size_t ladder = (size_t)ls * (size_t)d.po * (size_t)d.pv * sizeof(double) + (16ull << 20);
   return std::max(kGemmWorkspaceBytes, (ladder + 255) / 256 * 256);
 }
 
