@@ -720,9 +720,9 @@ int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const 
       const int pairs = (precision >> 16) & 1 ? 0 : -1;  // bit 16: element-wise loads only
       if ((precision >> 17) & 1) {  // bit 17: one level of Strassen (M = N = K even, beta 0)
         double* tmp = nullptr;
-        if (rc == CCSD_OK && (!strassen_applies(g, 2) ||
-                              hipMalloc(&tmp, strassen_temp_elems(M) * sizeof(double)) != hipSuccess))
-          rc = CCSD_ERR_ARG;
+        if (rc == CCSD_OK && !strassen_applies(g, 2)) rc = CCSD_ERR_ARG;
+        if (rc == CCSD_OK && hipMalloc(&tmp, strassen_temp_elems(M) * sizeof(double)) != hipSuccess)
+          rc = CCSD_ERR_HIP;
         if (rc == CCSD_OK && gemm_strassen(g, tmp, GemmWorkspace{ws, wsb}, s) != hipSuccess) rc = CCSD_ERR_HIP;
         if (rc == CCSD_OK && hipStreamSynchronize(s) != hipSuccess) rc = CCSD_ERR_HIP;
         if (tmp) (void)hipFree(tmp);
