@@ -11,7 +11,8 @@
 //   C11 = M1 + M2, C12 = M1 + M6 + M5 + M3, C21 = M1 + M6 + M7 - M4,
 //   C22 = M1 + M6 + M7 + M5
 // (V. Strassen, Numer. Math. 13, 354 (1969); S. Winograd's variant with 15
-// additions).  The schedule below accumulates in the C blocks and needs three
+// additions).  The schedule below accumulates in the C blocks (12 block
+// passes, three of them fused pairs) and needs three
 // h x h temporaries (S, T and one product).  Each product is one engine call at
 // the half size: at n = 12000 the seven 6000^3 products and the block passes
 // take less time than one 12000^3 product.  The error bound grows by a small
@@ -39,10 +40,42 @@ __global__ void block_axpby_kernel(double* D, long ldd, const double* X, long ld
   }
 }
 
+// D = X + Y and E = D in one pass (E may be null); or, with Y null, D += X and
+// E += X (the two accumulations of one block).
+__global__ void block_sum2_kernel(double* D, long ldd, double* E, long lde, const double* X, long ldx,
+                                  const double* Y, long ldy, long h) {
+  for (long r = blockIdx.y; r < h; r += gridDim.y) {
+    double* d = D + r * ldd;
+    double* e = E + r * lde;
+    const double* x = X + r * ldx;
+    const double* y = Y ? Y + r * ldy : nullptr;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < h; c += (long)gridDim.x * blockDim.x) {
+      if (y) {
+        const double v = x[c] + y[c];
+        d[c] = v;
+        e[c] = v;
+      } else {
+        const double v = x[c];
+        d[c] += v;
+        e[c] += v;
+      }
+    }
+  }
+}
+
 struct View {
   const double* p;
   long ld;
 };
+
+// (D, E) = (X + Y, X + Y), or with Y null (D + X, E + X)
+hipError_t sum2(View D, View E, View X, View Y, long h, hipStream_t s) {
+  const unsigned gx = (unsigned)std::min<long>((h + 255) / 256, 8);
+  const unsigned gy = (unsigned)std::min<long>(h, 65535);
+  hipLaunchKernelGGL(block_sum2_kernel, dim3(gx, gy), dim3(256), 0, s, const_cast<double*>(D.p), D.ld,
+                     const_cast<double*>(E.p), E.ld, X.p, X.ld, Y.p, Y.ld, h);
+  return hipGetLastError();
+}
 
 hipError_t axpby(View D, View X, double a, View Y, double b, long h, hipStream_t s) {
   const unsigned gx = (unsigned)std::min<long>((h + 255) / 256, 8);
@@ -107,20 +140,17 @@ hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorks
   ST(axpby(T, B22, 1.0, T, -1.0, h, s));     // T2
   ST(mul(C12, S, T, al, 0.0));               // M6
   ST(mul(C11, A11, B11, al, 0.0));           // M1
-  ST(axpby(C12, C12, 1.0, C11, 1.0, h, s));  // U2 = M1 + M6
+  ST(sum2(C12, C21, C12, C11, h, s));        // C12 = C21 = U2 = M1 + M6
   ST(mul(C11, A12, B21, al, 1.0));           // C11 = M1 + M2
   ST(axpby(S, A12, 1.0, S, -1.0, h, s));     // S4
   ST(axpby(T, T, 1.0, B21, -1.0, h, s));     // T4
-  ST(axpby(C21, C12, 1.0, View{nullptr, 0}, 0.0, h, s));  // C21 = U2
-  ST(axpby(C12, C12, 1.0, Tm, 1.0, h, s));   // U4 = U2 + M5
-  ST(axpby(C22, C12, 1.0, View{nullptr, 0}, 0.0, h, s));  // C22 = U4
+  ST(sum2(C12, C22, C12, Tm, h, s));         // C12 = C22 = U4 = U2 + M5
   ST(mul(C12, S, B22, al, 1.0));             // C12 = U4 + M3
   ST(mul(C21, A22, T, -al, 1.0));            // C21 = U2 - M4
   ST(axpby(S, A11, 1.0, A21, -1.0, h, s));   // S3
   ST(axpby(T, B22, 1.0, B12, -1.0, h, s));   // T3
   ST(mul(Tm, S, T, al, 0.0));                // M7
-  ST(axpby(C21, C21, 1.0, Tm, 1.0, h, s));   // C21 = U2 - M4 + M7
-  ST(axpby(C22, C22, 1.0, Tm, 1.0, h, s));   // C22 = U4 + M7
+  ST(sum2(C21, C22, Tm, View{nullptr, 0}, h, s));  // C21 = U2 - M4 + M7, C22 = U4 + M7
 #undef ST
   return hipSuccess;
 }
