@@ -546,8 +546,10 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
   if (lowp) {
     TRY(lowprec_adj_ring(c, A->Tringb, A->WTb, lam2));
   } else {
-    if (need_t) TRY(run_gemm(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
-    TRY(run_gemm(c, G64(A->Xb, 1, ov, c->Tring, ov, 1, A->WTb, ov, 1, ov, ov, ov, 1.0, 1.0), W));
+    // the (OV)^3 products go through square_product (one level of Strassen at
+    // OV >= 8192) with the forward's X as scratch: the reverse pass never reads X
+    if (need_t) TRY(square_product(c, G64(A->Xb, ov, 1, c->WT, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 1.0, 1.0), c->X, W));
+    TRY(square_product(c, G64(A->Xb, 1, ov, c->Tring, ov, 1, A->WTb, ov, 1, ov, ov, ov, 1.0, 1.0), c->X, W));
   }
 
   // -- Rp = <ab||ij> + tau . vvvv + W1^T . tau     (cotangent lam2)
@@ -604,8 +606,9 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- W_mbej (WT): (c) + 1/2 Tring . Doovv
   if (lowp) TRY(lowprec_adj_wmbej(c, A->Tringb, A->WTb));
-  else if (need_t) TRY(run_gemm(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), W));
-  if (th) TRY(run_gemm(c, G64(c->Tring, ov, 1, A->WTb, ov, 1, th->Doovv, ov, 1, ov, ov, ov, 0.5, 1.0), W));
+  else if (need_t)
+    TRY(square_product(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), c->X, W));
+  if (th) TRY(square_product(c, G64(c->Tring, ov, 1, A->WTb, ov, 1, th->Doovv, ov, 1, ov, ov, ov, 0.5, 1.0), c->X, W));
   // (b) + t1[n,b] W3u[j][m][n][e], batched over (j, m)
   if (need_t) {
     GemmDesc<double> g = G64(c->W3u, V, 1, A->WTb, 1, ov, t1b, V, 1, O, V, V, 1.0, 1.0);

@@ -286,17 +286,17 @@ static int run_f32(Ctx* c, const GemmDesc<float>& g) {
 
 // The (OV)^3 FP64 products: one level of Strassen (strassen.hip) when the
 // context holds its temporaries, else the classical engine call.  With
-// beta != 0 the Strassen product goes to `scratch` (OV x OV) first and is
-// added to C after.
-static int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch) {
+// beta == 1 the Strassen product goes to `scratch` (M x N, contiguous) first
+// and is added to C after.  ws: the engine workspace (the context's if null).
+int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws) {
   GemmDesc<double> h = g;
   if (g.beta != 0.0) {
     h.C = scratch;
     h.beta = 0.0;
   }
   if (!c->strassen_tmp || (g.beta != 0.0 && (!scratch || g.beta != 1.0 || g.sCm != g.N)) || !strassen_applies(h))
-    return run_gemm(c, g);
-  hipError_t e = gemm_strassen(h, c->strassen_tmp, c->ws, c->stream);
+    return run_gemm(c, g, ws);
+  hipError_t e = gemm_strassen(h, c->strassen_tmp, ws ? *ws : c->ws, c->stream);
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;

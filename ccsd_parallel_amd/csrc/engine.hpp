@@ -79,6 +79,10 @@ int ao_to_mo_device(double* eri, double* tmp, const double* C, int n, const Gemm
 int setup_synthetic(Ctx* c, uint64_t seed, double scale);
 int plan_and_allocate(Ctx* c, int precision);
 int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool time_it);
+// C = alpha A B + beta C for the square (OV)^3 FP64 products (beta 0 or 1):
+// one level of Strassen when the context holds its temporaries and the
+// product qualifies (with beta 1 through `scratch`, M x N), else the engine.
+int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws = nullptr);
 int device_energy(Ctx* c, const double* t1, const double* t2, double* e);
 size_t plan_bytes(int O, int V, int precision, int diis_m);
 // Bordered Pulay system (B_ij = <e_i|e_j>, -1 border, rhs -1) with diagonal
