@@ -42,6 +42,7 @@ class Options(C.Structure):
         ("diis_dim", C.c_int), ("diis_start", C.c_int), ("recompute_v4", C.c_int), ("verbose", C.c_int),
         ("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("diis_reset_on_switch", C.c_int),
         ("switch_factor", C.c_double), ("lambda_switch_factor", C.c_double), ("lambda_tol", C.c_double),
+        ("strassen_min_order", C.c_int),
     ]
 
 

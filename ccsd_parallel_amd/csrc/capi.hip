@@ -259,6 +259,7 @@ int ccsd_default_options(ccsd_options* o) {
   // iterations (2.64 s backward), 100 gives 13 + 6 (2.84 s), 300 gives 12 + 6 (2.92 s)
   o->lambda_switch_factor = 30.0;
   o->lambda_tol = 0.0;
+  o->strassen_min_order = 0;
   return CCSD_OK;
 }
 

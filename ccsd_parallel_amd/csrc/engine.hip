@@ -62,7 +62,15 @@ static size_t workspace_bytes(const Dims& d) {
   return std::max(kGemmWorkspaceBytes, (ladder + 255) / 256 * 256);
 }
 
-size_t plan_bytes(int O, int V, int precision, int diis_m) {
+// Smallest OV at which the (OV)^3 FP64 products take one level of Strassen
+// (ccsd_options.strassen_min_order: 0 = kStrassenMinN, < 0 = never).
+static bool uses_strassen(const Dims& d, int min_order) {
+  if (min_order < 0) return false;
+  const long m = min_order == 0 ? kStrassenMinN : std::max(min_order, 2);
+  return d.ov % 2 == 0 && d.ov >= m;
+}
+
+size_t plan_bytes(int O, int V, int precision, int diis_m, int strassen_min_order) {
   Dims d = make_dims(O, V);
   const long N = O + V;
   const long VV = (long)V * V;
@@ -83,7 +91,7 @@ size_t plan_bytes(int O, int V, int precision, int diis_m) {
   n += d.ov * d.ov;
   size_t bytes = (size_t)n * sizeof(double);
   bytes += workspace_bytes(d);  // GEMM workspace
-  if (d.ov % 2 == 0 && d.ov >= kStrassenMinN) bytes += strassen_temp_elems(d.ov) * sizeof(double);
+  if (uses_strassen(d, strassen_min_order)) bytes += strassen_temp_elems(d.ov) * sizeof(double);
   bytes += 1ull << 20;   // reductions
   bytes += lowprec_plan_bytes(d, precision);
   return bytes;
@@ -106,7 +114,7 @@ static int dalloc(Ctx* c, double** p, long n) {
 int plan_and_allocate(Ctx* c, int precision) {
   const Dims& d = c->d;
   const long O = d.O, V = d.V, N = O + V, VV = V * V;
-  size_t need = plan_bytes(d.O, d.V, precision, c->opt.diis_dim);
+  size_t need = plan_bytes(d.O, d.V, precision, c->opt.diis_dim, c->opt.strassen_min_order);
   size_t budget = c->opt.hbm_budget_bytes;
   if (budget == 0) {
     size_t fr = 0, tot = 0;
@@ -166,7 +174,8 @@ int plan_and_allocate(Ctx* c, int precision) {
   double* ws = nullptr;
   const size_t wsb = workspace_bytes(d);
   TRY(dalloc(c, &ws, (long)(wsb / 8)));
-  if (d.ov % 2 == 0 && d.ov >= kStrassenMinN) TRY(dalloc(c, &c->strassen_tmp, (long)strassen_temp_elems(d.ov)));
+  if (uses_strassen(d, c->opt.strassen_min_order))
+    TRY(dalloc(c, &c->strassen_tmp, (long)strassen_temp_elems(d.ov)));
   c->ws.ptr = ws;
   c->ws.bytes = wsb;
   // Low-precision operand copies are made on first use (lowprec_amplitudes),
@@ -285,7 +294,8 @@ static int run_f32(Ctx* c, const GemmDesc<float>& g) {
 }
 
 // The (OV)^3 FP64 products: one level of Strassen (strassen.hip) when the
-// context holds its temporaries, else the classical engine call.  With
+// context holds its temporaries (sized for order OV), else the classical
+// engine call.  With
 // beta == 1 the Strassen product goes to `scratch` (M x N, contiguous) first
 // and is added to C after.  ws: the engine workspace (the context's if null).
 int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws) {
@@ -294,7 +304,8 @@ int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const Gem
     h.C = scratch;
     h.beta = 0.0;
   }
-  if (!c->strassen_tmp || (g.beta != 0.0 && (!scratch || g.beta != 1.0 || g.sCm != g.N)) || !strassen_applies(h))
+  if (!c->strassen_tmp || g.M != c->d.ov || (g.beta != 0.0 && (!scratch || g.beta != 1.0 || g.sCm != g.N)) ||
+      !strassen_applies(h, 2))
     return run_gemm(c, g, ws);
   hipError_t e = gemm_strassen(h, c->strassen_tmp, ws ? *ws : c->ws, c->stream);
   if (e != hipSuccess) {

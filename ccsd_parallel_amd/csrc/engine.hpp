@@ -84,7 +84,8 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool 
 // product qualifies (with beta 1 through `scratch`, M x N), else the engine.
 int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws = nullptr);
 int device_energy(Ctx* c, const double* t1, const double* t2, double* e);
-size_t plan_bytes(int O, int V, int precision, int diis_m);
+// strassen_min_order: as ccsd_options (0 = the default order)
+size_t plan_bytes(int O, int V, int precision, int diis_m, int strassen_min_order = 0);
 // Bordered Pulay system (B_ij = <e_i|e_j>, -1 border, rhs -1) with diagonal
 // scaling; false when singular or the coefficients are not finite / huge.
 bool diis_solve_bordered(const std::vector<double>& B, int m, std::vector<double>& coef);

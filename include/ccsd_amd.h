@@ -86,6 +86,11 @@ typedef struct ccsd_options {
                                0 (default) = 1e-2 x the tol of the last ccsd_solve, clamped to
                                [1e-11, 1e-8] (the gradient is then as accurate as the amplitudes it
                                is taken at) */
+  int strassen_min_order;   /* FP64 mode: the square (OV)^3 products (the ring, W_mbej's quadratic term
+                               and their reverse products in ccsd_backward) use one level of
+                               Strassen-Winograd (seven half-order products, three (OV/2)^2 temporaries
+                               counted in the plan) when OV is even and at least this; 0 (default) =
+                               8192, < 0 = never (classical products throughout) */
 } ccsd_options;
 
 typedef struct ccsd_problem {
@@ -225,7 +230,8 @@ int ccsd_update_integrals(ccsd_ctx* c, int block, long n, const long* index, con
 int ccsd_host_alloc(size_t bytes, void** out);
 int ccsd_host_free(void* p);
 
-/* Device bytes the context holds (or would hold for O, V before create). */
+/* Device bytes a context with the default options holds (or would hold for O, V before
+ * create): DIIS(8), Strassen temporaries from OV = 8192. */
 size_t ccsd_plan_bytes(int n_occ, int n_virt, int precision);
 
 /* Per-kernel-class timing of the last iteration (milliseconds), HIP events on

@@ -34,10 +34,22 @@ def _prob(O, V, seed, noncanon=False):
 @pytest.mark.parametrize("O,V,seed,noncanon", [(3, 5, 35, True), (2, 3, 36, False), (4, 8, 31, False), (6, 12, 32, True),
                                                (8, 16, 33, False), (12, 40, 38, True), (14, 60, 39, False)])
 def test_backward_matches_cpu_adjoint(O, V, seed, noncanon):
+    _check_backward(O, V, seed, noncanon)
+
+
+# strassen_min_order 2: the forward's (OV)^3 products and the four reverse
+# ones (the ring's two cotangents, W_mbej's quadratic term and its
+# t-cotangent) through one level of Strassen, half orders 3, 36 and 240
+@pytest.mark.parametrize("O,V,seed,noncanon", [(2, 3, 36, False), (6, 12, 32, True), (12, 40, 38, True)])
+def test_backward_strassen_matches_cpu_adjoint(O, V, seed, noncanon):
+    _check_backward(O, V, seed, noncanon, strassen_min_order=2)
+
+
+def _check_backward(O, V, seed, noncanon, **opts):
     capi = _capi()
     prob = _prob(O, V, seed, noncanon)
     E, gref, _, linfo = adjoint_ref.adjoint(prob, toler=1e-13)
-    with capi.Context.from_problem(prob, diis_dim=8) as ctx:
+    with capi.Context.from_problem(prob, diis_dim=8, **opts) as ctx:
         r = ctx.solve(1e-13, 300)
         g, info = ctx.backward()
     assert abs(r["e_corr"] - E) < 1e-11

@@ -181,6 +181,40 @@ def test_gemm_lowprec(prec, tol, M, N, K):
 @pytest.mark.parametrize("O,V,seed", [(2, 2, 17), (3, 5, 18), (5, 3, 19), (2, 3, 20), (4, 8, 11), (6, 10, 12),
                                       (8, 20, 13), (10, 26, 14), (12, 40, 15), (14, 60, 16)])
 def test_residual_parity(O, V, seed):
+    _check_residual(O, V, seed)
+
+
+# strassen_min_order 2: the ring and W_mbej's quadratic term (both OV x OV x OV)
+# through one level of Strassen at every even OV, here with half orders 3, 16,
+# 30 (ragged against every tile), 240 and 420
+@pytest.mark.parametrize("O,V,seed", [(2, 3, 20), (4, 8, 11), (6, 10, 12), (12, 40, 15), (14, 60, 16)])
+def test_residual_parity_strassen(O, V, seed):
+    _check_residual(O, V, seed, strassen_min_order=2)
+
+
+def test_strassen_option_takes_effect():
+    """strassen_min_order reaches the plan and the products: with it the plan
+    grows by the three (OV/2)^2 temporaries (a budget of exactly the default
+    plan is refused, CCSD_ERR_BUDGET), and the residual differs from the
+    classical one in its last bits only."""
+    capi = _ctx()
+    O, V, seed = 12, 40, 15
+    prob = synth.make_problem(O, V, seed, scale=0.05)
+    plan = capi.plan_bytes(O, V, "fp64")  # default options: no Strassen at OV = 480
+    with pytest.raises(capi.CCSDError) as e:
+        capi.Context.from_problem(prob, diis_dim=8, hbm_budget_bytes=plan, strassen_min_order=2)
+    assert e.value.code == -3
+    t1, t2 = random_amplitudes(O, V, seed)
+    res = {}
+    for order in (-1, 2):
+        with capi.Context.from_problem(prob, diis_dim=8, hbm_budget_bytes=plan + (8 << 20),
+                                       strassen_min_order=order) as ctx:
+            res[order] = np.concatenate([r.ravel() for r in ctx.residual(t1, packed(t2, O, V))])
+    diff = np.abs(res[2] - res[-1]).max()
+    assert 0.0 < diff <= 1e-13 * np.abs(res[-1]).max(), diff
+
+
+def _check_residual(O, V, seed, **opts):
     capi = _ctx()
     prob = synth.make_problem(O, V, seed, scale=0.05)
     rng = np.random.default_rng(seed)
@@ -189,7 +223,7 @@ def test_residual_parity(O, V, seed):
     I = ccsd_ref.Ints.from_problem(prob)
     t1, t2 = random_amplitudes(O, V, seed)
     R1, R2, _, _, X = ccsd_ref.rhs(I, t1, t2, parts=True)
-    with capi.Context.from_problem(prob, diis_dim=0) as ctx:
+    with capi.Context.from_problem(prob, diis_dim=0, **opts) as ctx:
         r1, r2 = ctx.residual(t1, packed(t2, O, V))
         scale = max(np.abs(R2).max(), np.abs(R1).max())
         # intermediates first, so a failure names the term

@@ -85,6 +85,7 @@ def test_strerror_and_defaults():
     o = capi.default_options()
     assert o.diis_dim == 8 and o.final_fp64_sweeps == 3 and o.precision == 0
     assert o.switch_factor == 100.0 and o.switch_threshold == 0.0 and o.diis_reset_on_switch == 1
+    assert o.strassen_min_order == 0 and o.lambda_tol == 0.0  # the last two fields: the struct sizes agree
 
 
 def test_plan_fits_hbm():
