@@ -370,8 +370,8 @@ def selftest_gemm(A, B, C_in, alpha=1.0, beta=0.0, ta=False, tb=False, precision
     """C = alpha op(A) op(B) + beta C through the device GEMM engine; tile > 0
     forces that tile configuration (index + 1) instead of the automatic choice;
     elementwise forces single-element global loads where the FP64 engine would
-    load element pairs; strassen runs one level of Strassen's method (FP64,
-    square, even size, beta 0)."""
+    load element pairs; strassen runs one level of Strassen's method (FP64 or
+    FP32, square, even size, beta 0)."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     Cm = np.ascontiguousarray(C_in, dtype=np.float64).copy()

@@ -10,6 +10,7 @@
 
 #include "bf16.hpp"
 #include "lowprec.hpp"
+#include "strassen.hpp"
 
 namespace ccsd {
 
@@ -749,9 +750,34 @@ int lowprec_r1_ring(Ctx* c, int prec) {
 // operands; in BF16 mode the product can also be emitted as an interleaved
 // hi/lo operand (Ob, leading dimension ldo).  With C null the FP32 C32 is the
 // destination instead (in BF16 mode stored back only if keep32).
+// FP32 square (OV)^3 product: one level of Strassen (strassen.hip) when the
+// context holds its temporaries, else the engine.  With beta 1 (FP32 result)
+// the Strassen product goes to `scratch` (M x N FP32, contiguous) and is added
+// after.
+static hipError_t f32_square(Ctx* c, const GemmDesc<float>& g, float* scratch) {
+  if (c->strassen_tmp && g.M == c->d.ov) {
+    GemmDesc<float> h = g;
+    const bool acc = g.beta != 0.0f;
+    if (acc) {
+      h.C = scratch;
+      h.Cd = nullptr;
+      h.sCm = g.N;
+      h.beta = 0.0f;
+    }
+    if ((!acc || (scratch && g.C && !g.Cd && g.beta == 1.0f)) && strassen_applies(h, 2)) {
+      hipError_t e = gemm_strassen(h, c->strassen_tmp, c->ws, c->stream);
+      if (e == hipSuccess && acc) e = strassen_accumulate(g.C, g.sCm, scratch, g.N, g.M, g.N, c->stream);
+      return e;
+    }
+  }
+  return gemm<float>(g, c->ws, c->stream);
+}
+
+// square: an (OV)^3 product (FP32: through f32_square, with `scratch` for beta 1)
 static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, long lda, const float* B32,
                       const uint16_t* Bb, long ldb, int M, int N, int K, double* C, long ldc, double alpha,
-                      double beta, uint16_t* Ob = nullptr, long ldo = 0, float* C32 = nullptr, bool keep32 = true) {
+                      double beta, uint16_t* Ob = nullptr, long ldo = 0, float* C32 = nullptr, bool keep32 = true,
+                      bool square = false, float* scratch = nullptr) {
   hipError_t e;
   if (prec == CCSD_PREC_FP32) {
     GemmDesc<float> g;
@@ -760,7 +786,7 @@ static int nt_product(Ctx* c, int prec, const float* A32, const uint16_t* Ab, lo
     g.M = M; g.N = N; g.K = K;
     g.sAm = lda; g.sAk = 1; g.sBk = 1; g.sBn = ldb; g.sCm = ldc; g.sCn = 1;
     g.alpha = (float)alpha; g.beta = (float)beta;  // 1, 0.5, 0: exact in FP32
-    e = gemm<float>(g, c->ws, c->stream);
+    e = square ? f32_square(c, g, scratch) : gemm<float>(g, c->ws, c->stream);
   } else {
     Bf16Out o;
     o.C64 = C; o.ldc = ldc; o.alpha = alpha; o.beta = beta;
@@ -799,10 +825,12 @@ int lowprec_wmbej(Ctx* c, int prec) {
   long ld;
   float* W = lowprec_wmbej_acc(c, prec, &ld);
   // Doovv is symmetric, so its rows serve as the k-contiguous B operand.  FP32:
-  // W is the ring GEMM's operand itself.  BF16: the finished W_mbej is written
-  // only as the ring GEMM's hi/lo operand.
+  // W is the ring GEMM's operand itself, and X (free until the ring GEMM)
+  // holds the Strassen product before it is added.  BF16: the finished W_mbej
+  // is written only as the ring GEMM's hi/lo operand.
   return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->Doovv32, L->Doovv_b, L->ld_ov, (int)d.ov,
-                    (int)d.ov, (int)d.ov, nullptr, ld, 0.5, 1.0, L->WT_b, L->ld_ov, W, false);
+                    (int)d.ov, (int)d.ov, nullptr, ld, 0.5, 1.0, L->WT_b, L->ld_ov, W, false, true,
+                    reinterpret_cast<float*>(c->X));
 }
 
 int lowprec_ladder(Ctx* c, int prec) {
@@ -819,7 +847,7 @@ int lowprec_ring(Ctx* c, int prec) {
   const Dims& d = c->d;
   // the W_mbej operand (WT32 / WT_b) is final after lowprec_wmbej
   return nt_product(c, prec, L->Tring32, L->Tring_b, L->ld_ov, L->WT32, L->WT_b, L->ld_ov, (int)d.ov, (int)d.ov,
-                    (int)d.ov, c->X, d.ov, 1.0, 0.0);
+                    (int)d.ov, c->X, d.ov, 1.0, 0.0, nullptr, 0, nullptr, true, true);
 }
 
 // ---------------------------------------------------------------- adjoint (Lambda) pieces
@@ -1006,6 +1034,7 @@ GemmDesc<float> f32_desc(const GemmDesc<double>& g, const float* A32, const floa
 int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K, double alpha, double beta,
                     int prec, hipStream_t s) {
   const int tile = (prec >> 8) & 0xff;  // 0: automatic, else forced tile configuration + 1
+  const bool strassen = (prec >> 17) & 1;  // FP32: one level of Strassen (M = N = K even, beta 0)
   prec &= 0xff;
   const long ld = pad64(K);
   float *A32 = nullptr, *B32 = nullptr;
@@ -1039,7 +1068,20 @@ int lowprec_gemm_nt(const double* A, const double* B, double* C, int M, int N, i
       g.sAm = ld; g.sAk = 1; g.sBk = 1; g.sBn = ld; g.sCm = N; g.sCn = 1;
       g.alpha = (float)alpha; g.beta = (float)beta;
       GemmWorkspace ws{wsp, kWs};
-      e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
+      if (strassen) {
+        if (!strassen_applies(g, 2)) {
+          for (void* q : owned) (void)hipFree(q);
+          return CCSD_ERR_ARG;
+        }
+        double* tmp = nullptr;
+        e = al((void**)&tmp, strassen_temp_elems(M) * sizeof(double));
+        if (e == hipSuccess) e = gemm_strassen(g, tmp, ws, s);
+      } else {
+        e = gemm<float>(g, ws, s, nullptr, tile - 1, -1);
+      }
+    } else if (strassen) {
+      for (void* q : owned) (void)hipFree(q);
+      return CCSD_ERR_ARG;
     } else {
       hipLaunchKernelGGL(conv_bf16il_kernel<double>, dim3(grid_for(M * ld / 8)), dim3(kThreads), 0, s, Ab, A, (long)M,
                          (long)K, (long)K, ld, 1.0);

@@ -1,7 +1,8 @@
 // One level of Strassen's method (Winograd's form: 7 half-size products, 15
-// block additions) over the MFMA GEMM engine, for the square FP64 products of
-// the spin-orbital iteration (the ring Tring . W_mbej^T and the W_mbej
-// quadratic term, both (OV)^3).
+// block additions) over the MFMA GEMM engine, for the square products of the
+// spin-orbital iteration (the ring Tring . W_mbej^T and the W_mbej quadratic
+// term, both (OV)^3), in FP64 and on FP32 operands (with an FP32 or an FP64
+// result).
 //
 // With A, B, C split into h x h blocks (h = n / 2):
 //   S1 = A21 + A22, S2 = S1 - A11, S3 = A11 - A21, S4 = A12 - S2
@@ -13,13 +14,15 @@
 // (V. Strassen, Numer. Math. 13, 354 (1969); S. Winograd's variant with 15
 // additions).  The schedule below accumulates in the C blocks (12 block
 // passes, three of them fused pairs) and needs three
-// h x h temporaries (S, T and one product).  Each product is one engine call at
+// h x h temporaries (S and T of the operand type, one product of the result
+// type).  Each product is one engine call at
 // the half size: at n = 12000 the seven 6000^3 products and the block passes
 // take less time than one 12000^3 product.  The error bound grows by a small
 // constant factor over the classical product (one level only).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "gemm.hpp"
 #include "strassen.hpp"
@@ -27,35 +30,38 @@
 namespace ccsd {
 namespace {
 
-// D = a X + b Y over an h x h block; each view is (pointer, row stride) with
-// unit stride along its columns.  Y may be null (D = a X).  D may alias X or Y.
-__global__ void block_axpby_kernel(double* D, long ldd, const double* X, long ldx, double a, const double* Y,
-                                   long ldy, double b, long h) {
-  for (long r = blockIdx.y; r < h; r += gridDim.y) {
-    double* d = D + r * ldd;
-    const double* x = X + r * ldx;
-    const double* y = Y ? Y + r * ldy : nullptr;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < h; c += (long)gridDim.x * blockDim.x)
+// D = a X + b Y over a rows x cols block; each view is (pointer, row stride)
+// with unit stride along its columns.  Y may be null (D = a X).  D may alias X
+// or Y.
+template <typename T>
+__global__ void block_axpby_kernel(T* D, long ldd, const T* X, long ldx, T a, const T* Y, long ldy, T b, long rows,
+                                   long cols) {
+  for (long r = blockIdx.y; r < rows; r += gridDim.y) {
+    T* d = D + r * ldd;
+    const T* x = X + r * ldx;
+    const T* y = Y ? Y + r * ldy : nullptr;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < cols; c += (long)gridDim.x * blockDim.x)
       d[c] = y ? a * x[c] + b * y[c] : a * x[c];
   }
 }
 
 // D = X + Y and E = D in one pass (E may be null); or, with Y null, D += X and
 // E += X (the two accumulations of one block).
-__global__ void block_sum2_kernel(double* D, long ldd, double* E, long lde, const double* X, long ldx,
-                                  const double* Y, long ldy, long h) {
+template <typename T>
+__global__ void block_sum2_kernel(T* D, long ldd, T* E, long lde, const T* X, long ldx, const T* Y, long ldy,
+                                  long h) {
   for (long r = blockIdx.y; r < h; r += gridDim.y) {
-    double* d = D + r * ldd;
-    double* e = E + r * lde;
-    const double* x = X + r * ldx;
-    const double* y = Y ? Y + r * ldy : nullptr;
+    T* d = D + r * ldd;
+    T* e = E + r * lde;
+    const T* x = X + r * ldx;
+    const T* y = Y ? Y + r * ldy : nullptr;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < h; c += (long)gridDim.x * blockDim.x) {
       if (y) {
-        const double v = x[c] + y[c];
+        const T v = x[c] + y[c];
         d[c] = v;
         e[c] = v;
       } else {
-        const double v = x[c];
+        const T v = x[c];
         d[c] += v;
         e[c] += v;
       }
@@ -63,40 +69,42 @@ __global__ void block_sum2_kernel(double* D, long ldd, double* E, long lde, cons
   }
 }
 
+template <typename T>
 struct View {
-  const double* p;
+  const T* p;
   long ld;
 };
 
+dim3 block_grid(long rows, long cols) {
+  return dim3((unsigned)std::min<long>((cols + 255) / 256, 8), (unsigned)std::min<long>(rows, 65535));
+}
+
 // (D, E) = (X + Y, X + Y), or with Y null (D + X, E + X)
-hipError_t sum2(View D, View E, View X, View Y, long h, hipStream_t s) {
-  const unsigned gx = (unsigned)std::min<long>((h + 255) / 256, 8);
-  const unsigned gy = (unsigned)std::min<long>(h, 65535);
-  hipLaunchKernelGGL(block_sum2_kernel, dim3(gx, gy), dim3(256), 0, s, const_cast<double*>(D.p), D.ld,
-                     const_cast<double*>(E.p), E.ld, X.p, X.ld, Y.p, Y.ld, h);
+template <typename T>
+hipError_t sum2(View<T> D, View<T> E, View<T> X, View<T> Y, long h, hipStream_t s) {
+  hipLaunchKernelGGL(block_sum2_kernel<T>, block_grid(h, h), dim3(256), 0, s, const_cast<T*>(D.p), D.ld,
+                     const_cast<T*>(E.p), E.ld, X.p, X.ld, Y.p, Y.ld, h);
   return hipGetLastError();
 }
 
-hipError_t axpby(View D, View X, double a, View Y, double b, long h, hipStream_t s) {
-  const unsigned gx = (unsigned)std::min<long>((h + 255) / 256, 8);
-  const unsigned gy = (unsigned)std::min<long>(h, 65535);
-  hipLaunchKernelGGL(block_axpby_kernel, dim3(gx, gy), dim3(256), 0, s, const_cast<double*>(D.p), D.ld, X.p, X.ld, a,
-                     Y.p, Y.ld, b, h);
+template <typename T>
+hipError_t axpby(View<T> D, View<T> X, T a, View<T> Y, T b, long h, hipStream_t s) {
+  hipLaunchKernelGGL(block_axpby_kernel<T>, block_grid(h, h), dim3(256), 0, s, const_cast<T*>(D.p), D.ld, X.p, X.ld,
+                     a, Y.p, Y.ld, b, h, h);
   return hipGetLastError();
 }
 
-}  // namespace
-
-bool strassen_applies(const GemmDesc<double>& d, int min_n) {
+template <typename T>
+bool applies(const GemmDesc<T>& d, int min_n) {
   return d.M == d.N && d.N == d.K && d.M % 2 == 0 && d.M >= min_n && d.nb1 == 1 && d.nb2 == 1 &&
-         !d.reduce_batch && d.beta == 0.0 && d.sCn == 1 && d.Cd == nullptr && (d.sAk == 1 || d.sAm == 1) &&
-         (d.sBk == 1 || d.sBn == 1);
+         !d.reduce_batch && d.beta == T(0) && d.sCn == 1 && (d.sAk == 1 || d.sAm == 1) && (d.sBk == 1 || d.sBn == 1);
 }
 
-size_t strassen_temp_elems(long n) { return 3ull * (size_t)(n / 2) * (size_t)(n / 2); }
-
-hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s) {
-  if (!strassen_applies(d, 2) || !tmp) return hipErrorInvalidValue;
+// C = alpha A B on operands of type E; the result (type R) is d.C (R = E) or
+// d.Cd (R = double, E = float).  tmp: S and T (h^2 of E each), then the
+// product temporary (h^2 of R).
+template <typename E, typename R>
+hipError_t strassen_run(const GemmDesc<E>& d, R* Cr, void* tmp, const GemmWorkspace& ws, hipStream_t s) {
   const long h = d.M / 2;
   // Operand blocks are handled in the orientation that keeps their unit stride
   // along the columns of a view: A as [m][k] (k-contiguous) or [k][m]; B as
@@ -105,54 +113,86 @@ hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorks
   const bool ak = d.sAk == 1, bk = d.sBk == 1;
   const long lda = ak ? d.sAm : d.sAk, ldb = bk ? d.sBn : d.sBk;
   auto ablk = [&](int bi, int bj) {  // A block (bi, bj): rows m in bi, k in bj
-    return View{d.A + bi * h * d.sAm + bj * h * d.sAk, lda};
+    return View<E>{d.A + bi * h * d.sAm + bj * h * d.sAk, lda};
   };
   auto bblk = [&](int bi, int bj) {  // B block (bi, bj): k in bi, n in bj
-    return View{d.B + bi * h * d.sBk + bj * h * d.sBn, ldb};
+    return View<E>{d.B + bi * h * d.sBk + bj * h * d.sBn, ldb};
   };
-  auto cblk = [&](int bi, int bj) { return View{d.C + bi * h * d.sCm + bj * h, d.sCm}; };
-  double* Sp = tmp;
-  double* Tp = tmp + h * h;
-  double* Mp = tmp + 2 * h * h;
-  const View S{Sp, h}, T{Tp, h}, Tm{Mp, h};
+  auto cblk = [&](int bi, int bj) { return View<R>{Cr + bi * h * d.sCm + bj * h, d.sCm}; };
+  E* Sp = static_cast<E*>(tmp);
+  E* Tp = Sp + h * h;
+  R* Mp = reinterpret_cast<R*>(Sp + 2 * h * h);
+  const View<E> S{Sp, h}, T{Tp, h};
+  const View<R> Tm{Mp, h};
   // C(view) = alpha a b + beta C, a an A-shaped view, b a B-shaped view
-  auto mul = [&](View c, View a, View b, double alpha, double beta) {
-    GemmDesc<double> g;
-    g.A = a.p; g.B = b.p; g.C = const_cast<double*>(c.p);
+  auto mul = [&](View<R> c, View<E> a, View<E> b, E alpha, E beta) {
+    GemmDesc<E> g;
+    g.A = a.p; g.B = b.p;
+    if constexpr (std::is_same<E, R>::value)
+      g.C = const_cast<R*>(c.p);
+    else
+      g.Cd = const_cast<R*>(c.p);
     g.M = (int)h; g.N = (int)h; g.K = (int)h;
     if (ak) { g.sAm = a.ld; g.sAk = 1; } else { g.sAm = 1; g.sAk = a.ld; }
     if (bk) { g.sBk = 1; g.sBn = b.ld; } else { g.sBk = b.ld; g.sBn = 1; }
     g.sCm = c.ld; g.sCn = 1;
     g.alpha = alpha; g.beta = beta;
-    return gemm<double>(g, ws, s);
+    return gemm<E>(g, ws, s);
   };
-  const double al = d.alpha;
-  const View A11 = ablk(0, 0), A12 = ablk(0, 1), A21 = ablk(1, 0), A22 = ablk(1, 1);
-  const View B11 = bblk(0, 0), B12 = bblk(0, 1), B21 = bblk(1, 0), B22 = bblk(1, 1);
-  const View C11 = cblk(0, 0), C12 = cblk(0, 1), C21 = cblk(1, 0), C22 = cblk(1, 1);
+  const E al = d.alpha, one = 1, zero = 0;
+  const View<E> A11 = ablk(0, 0), A12 = ablk(0, 1), A21 = ablk(1, 0), A22 = ablk(1, 1);
+  const View<E> B11 = bblk(0, 0), B12 = bblk(0, 1), B21 = bblk(1, 0), B22 = bblk(1, 1);
+  const View<R> C11 = cblk(0, 0), C12 = cblk(0, 1), C21 = cblk(1, 0), C22 = cblk(1, 1);
   hipError_t e;
 #define ST(call) \
   if ((e = (call)) != hipSuccess) return e
-  ST(axpby(S, A21, 1.0, A22, 1.0, h, s));    // S1
-  ST(axpby(T, B12, 1.0, B11, -1.0, h, s));   // T1
-  ST(mul(Tm, S, T, al, 0.0));                // M5
-  ST(axpby(S, S, 1.0, A11, -1.0, h, s));     // S2
-  ST(axpby(T, B22, 1.0, T, -1.0, h, s));     // T2
-  ST(mul(C12, S, T, al, 0.0));               // M6
-  ST(mul(C11, A11, B11, al, 0.0));           // M1
+  ST(axpby(S, A21, one, A22, one, h, s));    // S1
+  ST(axpby(T, B12, one, B11, -one, h, s));   // T1
+  ST(mul(Tm, S, T, al, zero));               // M5
+  ST(axpby(S, S, one, A11, -one, h, s));     // S2
+  ST(axpby(T, B22, one, T, -one, h, s));     // T2
+  ST(mul(C12, S, T, al, zero));              // M6
+  ST(mul(C11, A11, B11, al, zero));          // M1
   ST(sum2(C12, C21, C12, C11, h, s));        // C12 = C21 = U2 = M1 + M6
-  ST(mul(C11, A12, B21, al, 1.0));           // C11 = M1 + M2
-  ST(axpby(S, A12, 1.0, S, -1.0, h, s));     // S4
-  ST(axpby(T, T, 1.0, B21, -1.0, h, s));     // T4
+  ST(mul(C11, A12, B21, al, one));           // C11 = M1 + M2
+  ST(axpby(S, A12, one, S, -one, h, s));     // S4
+  ST(axpby(T, T, one, B21, -one, h, s));     // T4
   ST(sum2(C12, C22, C12, Tm, h, s));         // C12 = C22 = U4 = U2 + M5
-  ST(mul(C12, S, B22, al, 1.0));             // C12 = U4 + M3
-  ST(mul(C21, A22, T, -al, 1.0));            // C21 = U2 - M4
-  ST(axpby(S, A11, 1.0, A21, -1.0, h, s));   // S3
-  ST(axpby(T, B22, 1.0, B12, -1.0, h, s));   // T3
-  ST(mul(Tm, S, T, al, 0.0));                // M7
-  ST(sum2(C21, C22, Tm, View{nullptr, 0}, h, s));  // C21 = U2 - M4 + M7, C22 = U4 + M7
+  ST(mul(C12, S, B22, al, one));             // C12 = U4 + M3
+  ST(mul(C21, A22, T, -al, one));            // C21 = U2 - M4
+  ST(axpby(S, A11, one, A21, -one, h, s));   // S3
+  ST(axpby(T, B22, one, B12, -one, h, s));   // T3
+  ST(mul(Tm, S, T, al, zero));               // M7
+  ST(sum2(C21, C22, Tm, View<R>{nullptr, 0}, h, s));  // C21 = U2 - M4 + M7, C22 = U4 + M7
 #undef ST
   return hipSuccess;
+}
+
+}  // namespace
+
+bool strassen_applies(const GemmDesc<double>& d, int min_n) { return applies(d, min_n) && d.Cd == nullptr; }
+
+bool strassen_applies(const GemmDesc<float>& d, int min_n) {
+  return applies(d, min_n) && ((d.C == nullptr) != (d.Cd == nullptr));
+}
+
+size_t strassen_temp_elems(long n) { return 3ull * (size_t)(n / 2) * (size_t)(n / 2); }
+
+hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s) {
+  if (!strassen_applies(d, 2) || !tmp) return hipErrorInvalidValue;
+  return strassen_run<double, double>(d, d.C, tmp, ws, s);
+}
+
+hipError_t gemm_strassen(const GemmDesc<float>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s) {
+  if (!strassen_applies(d, 2) || !tmp) return hipErrorInvalidValue;
+  if (d.Cd) return strassen_run<float, double>(d, d.Cd, tmp, ws, s);
+  return strassen_run<float, float>(d, d.C, tmp, ws, s);
+}
+
+hipError_t strassen_accumulate(float* C, long ldc, const float* P, long ldp, long rows, long cols, hipStream_t s) {
+  hipLaunchKernelGGL(block_axpby_kernel<float>, block_grid(rows, cols), dim3(256), 0, s, C, ldc, P, ldp, 1.0f, C,
+                     ldc, 1.0f, rows, cols);
+  return hipGetLastError();
 }
 
 }  // namespace ccsd

@@ -285,12 +285,13 @@ long ccsd_debug_get(ccsd_ctx* c, const char* name, double* out, long cap);
  * C(MxN) = alpha*A(MxK)*B(KxN) + beta*C, A/B/C row-major, op flags transpose
  * the stored A (ta) / B (tb); precision selects FP64 / FP32 / BF16X3 arithmetic.
  * Bits 8..15 of precision force a tile configuration (0 = automatic): for
- * FP64 / FP32 the generic MFMA engine's configuration index + 1 (1..14); for
+ * FP64 / FP32 the generic MFMA engine's configuration index + 1 (1..17); for
  * BF16X3 the hi/lo kernel's (1 = 128x128, 2 = 256x256 with the automatic
- * tile order, 3 = 256x256 plain order, 4 = 256x256 grouped order).  Bit 16
- * (FP64) forces single-element global loads where the engine would load
- * element pairs.  Bit 17 (FP64, M = N = K even, beta = 0) runs the product
- * through one level of Strassen's method (seven half-size engine calls). */
+ * tile order, 3 = 256x256 plain order, 4 = 256x256 grouped order, 5 =
+ * 160x320, 6 = 256x48).  Bit 16 (FP64) forces single-element global loads
+ * where the engine would load element pairs.  Bit 17 (FP64 or FP32, M = N = K
+ * even, beta = 0) runs the product through one level of Strassen's method
+ * (seven half-size engine calls). */
 int ccsd_selftest_gemm(int M, int N, int K, int ta, int tb, double alpha, const double* A, const double* B,
                        double beta, double* C, int precision);
 

@@ -73,6 +73,27 @@ def test_gemm_fp64_strassen(n, ta, tb):
         capi.selftest_gemm(A, B, C0, 0.7, 0.5, precision="fp64", strassen=True)
 
 
+@pytest.mark.parametrize("n", [64, 998, 1026])
+def test_gemm_fp32_strassen(n):
+    """One level of Strassen on FP32 operands with the FP64 result (the FP32
+    ring's form), half sizes 32, 499 and 513: its error against the FP64
+    product stays within a small factor of the classical FP32 product's."""
+    capi = _ctx()
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, n))
+    C0 = rng.standard_normal((n, n))
+    ref = 0.5 * (A @ B)
+    classical = capi.selftest_gemm(A, B, C0, 0.5, 0.0, precision="fp32")
+    out = capi.selftest_gemm(A, B, C0, 0.5, 0.0, precision="fp32", strassen=True)
+    e_c = np.max(np.abs(classical - ref))
+    e_s = np.max(np.abs(out - ref))
+    assert 0 < e_c and e_s <= 8 * e_c, (e_s, e_c)
+    for prec, beta in (("fp32", 0.5), ("bf16x3", 0.0)):  # beta != 0, and BF16x3, are refused
+        with pytest.raises(capi.CCSDError):
+            capi.selftest_gemm(A, B, C0, 0.5, beta, precision=prec, strassen=True)
+
+
 N_FP64_TILES = 17  # configurations of the generic MFMA engine
 N_BF16_TILES = 6   # configurations of the BF16 hi/lo kernel
 
@@ -409,10 +430,22 @@ def test_low_precision_update_matches_fp64_update(prec, tol, O, V, seed):
     FP64 and in low precision.  Every contraction of the low-precision residual
     -- the BF16/FP32 dominant GEMMs and the FP32 secondary ones -- enters the
     new amplitudes, which must agree to the rounding level of the mode."""
+    _check_low_precision_update(prec, tol, O, V, seed)
+
+
+# strassen_min_order 2: the FP32 ring (FP64 result) and W_mbej's quadratic term
+# (FP32 result accumulated into the FP32 W_mbej) through one level of Strassen,
+# half orders 144 and 500
+@pytest.mark.parametrize("O,V,seed", [(8, 36, 77), (50, 20, 80)])
+def test_fp32_update_with_strassen_matches_fp64_update(O, V, seed):
+    _check_low_precision_update("fp32", 5e-6, O, V, seed, strassen_min_order=2)
+
+
+def _check_low_precision_update(prec, tol, O, V, seed, **opts):
     capi = _ctx()
     sc = norm_scale(O, V)
     rng = np.random.default_rng(5)
-    with capi.Context.synthetic(O, V, seed, sc) as ctx:
+    with capi.Context.synthetic(O, V, seed, sc, **opts) as ctx:
         ctx.iterate(2)
         t1, t2 = ctx.get_amplitudes()
         t1 = t1 + 0.01 * rng.standard_normal(t1.shape)
