@@ -86,8 +86,8 @@ typedef struct ccsd_options {
                                0 (default) = 1e-2 x the tol of the last ccsd_solve, clamped to
                                [1e-11, 1e-8] (the gradient is then as accurate as the amplitudes it
                                is taken at) */
-  int strassen_min_order;   /* FP64 mode: the square (OV)^3 products (the ring, W_mbej's quadratic term
-                               and their reverse products in ccsd_backward) use one level of
+  int strassen_min_order;   /* FP64 and FP32 arithmetic: the square (OV)^3 products (the ring, W_mbej's
+                               quadratic term and their FP64 reverse products in ccsd_backward) use one level of
                                Strassen-Winograd (seven half-order products, three (OV/2)^2 temporaries
                                counted in the plan) when OV is even and at least this; 0 (default) =
                                8192, < 0 = never (classical products throughout) */
