@@ -606,8 +606,11 @@ static int reverse_pass(Ctx* c, const double* lam, ThetaBars* th, bool lowp = fa
 
   // -- W_mbej (WT): (c) + 1/2 Tring . Doovv
   if (lowp) TRY(lowprec_adj_wmbej(c, A->Tringb, A->WTb));
-  else if (need_t)
-    TRY(square_product(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), c->X, W));
+  else if (need_t) {
+    StrassenSums sd;  // B = Doovv with the forward W_mbej product's strides: its operand sums
+    sd.T = c->strassen_doovv_t;
+    TRY(square_product(c, G64(A->WTb, ov, 1, c->Doovv, ov, 1, A->Tringb, ov, 1, ov, ov, ov, 0.5, 1.0), c->X, W, &sd));
+  }
   if (th) TRY(square_product(c, G64(c->Tring, ov, 1, A->WTb, ov, 1, th->Doovv, ov, 1, ov, ov, ov, 0.5, 1.0), c->X, W));
   // (b) + t1[n,b] W3u[j][m][n][e], batched over (j, m)
   if (need_t) {

@@ -653,6 +653,7 @@ int ccsd_update_integrals(ccsd_ctx* h, int block, long n, const long* index, con
     if (e != hipSuccess) { c->last_error = hipGetErrorString(e); return CCSD_ERR_HIP; }
     if (block == 2) {  // derived <mn||ef> layouts
       k_build_Doovv(c->Doovv, c->oovv, d, s);
+      if (int rc = strassen_doovv_sums(c)) return rc;
       k_build_oovv_ou(c->oovv_ou, c->oovv, d, s);
       k_build_oovv_vu(c->oovv_vu, c->oovv, d, s);
     }

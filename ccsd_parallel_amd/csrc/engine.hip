@@ -91,7 +91,8 @@ size_t plan_bytes(int O, int V, int precision, int diis_m, int strassen_min_orde
   n += d.ov * d.ov;
   size_t bytes = (size_t)n * sizeof(double);
   bytes += workspace_bytes(d);  // GEMM workspace
-  if (uses_strassen(d, strassen_min_order)) bytes += strassen_temp_elems(d.ov) * sizeof(double);
+  if (uses_strassen(d, strassen_min_order))  // temporaries, Tring's and Doovv's operand sums
+    bytes += (strassen_temp_elems(d.ov) + 8 * (size_t)(d.ov / 2) * (size_t)(d.ov / 2)) * sizeof(double);
   bytes += 1ull << 20;   // reductions
   bytes += lowprec_plan_bytes(d, precision);
   return bytes;
@@ -174,8 +175,12 @@ int plan_and_allocate(Ctx* c, int precision) {
   double* ws = nullptr;
   const size_t wsb = workspace_bytes(d);
   TRY(dalloc(c, &ws, (long)(wsb / 8)));
-  if (uses_strassen(d, c->opt.strassen_min_order))
+  if (uses_strassen(d, c->opt.strassen_min_order)) {
+    const long hh = (d.ov / 2) * (d.ov / 2);
     TRY(dalloc(c, &c->strassen_tmp, (long)strassen_temp_elems(d.ov)));
+    TRY(dalloc(c, &c->strassen_tring_s, 4 * hh));
+    TRY(dalloc(c, &c->strassen_doovv_t, 4 * hh));
+  }
   c->ws.ptr = ws;
   c->ws.bytes = wsb;
   // Low-precision operand copies are made on first use (lowprec_amplitudes),
@@ -191,6 +196,7 @@ static int finish_static(Ctx* c, const double* ovov_dev) {
   hipStream_t s = c->stream;
   k_build_G(c->G, ovov_dev, d, s);
   k_build_Doovv(c->Doovv, c->oovv, d, s);
+  TRY(strassen_doovv_sums(c));
   k_build_ovvv_u(c->ovvv_u, c->ovvv, d, s);
   k_build_oovv_ou(c->oovv_ou, c->oovv, d, s);
   k_build_oovv_vu(c->oovv_vu, c->oovv, d, s);
@@ -298,7 +304,8 @@ static int run_f32(Ctx* c, const GemmDesc<float>& g) {
 // engine call.  With
 // beta == 1 the Strassen product goes to `scratch` (M x N, contiguous) first
 // and is added to C after.  ws: the engine workspace (the context's if null).
-int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws) {
+int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws,
+                   const StrassenSums* sums) {
   GemmDesc<double> h = g;
   if (g.beta != 0.0) {
     h.C = scratch;
@@ -307,12 +314,29 @@ int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const Gem
   if (!c->strassen_tmp || g.M != c->d.ov || (g.beta != 0.0 && (!scratch || g.beta != 1.0 || g.sCm != g.N)) ||
       !strassen_applies(h, 2))
     return run_gemm(c, g, ws);
-  hipError_t e = gemm_strassen(h, c->strassen_tmp, ws ? *ws : c->ws, c->stream);
+  hipError_t e = gemm_strassen(h, c->strassen_tmp, ws ? *ws : c->ws, c->stream, sums);
   if (e != hipSuccess) {
     c->last_error = hipGetErrorString(e);
     return CCSD_ERR_HIP;
   }
   if (g.beta != 0.0) k_axpy(g.C, scratch, 1.0, (long)g.M * g.N, c->stream);
+  return CCSD_OK;
+}
+
+// The W_mbej quadratic term's product Tring . Doovv (B = Doovv, k-major rows):
+// the descriptor the operand sums are formed for and used with.
+static GemmDesc<double> wmbej_quad_desc(Ctx* c, double beta) {
+  const long ov = c->d.ov;
+  return G64(c->Tring, ov, 1, c->Doovv, ov, 1, c->WT, ov, 1, ov, ov, ov, 0.5, beta);
+}
+
+int strassen_doovv_sums(Ctx* c) {
+  if (!c->strassen_doovv_t) return CCSD_OK;
+  hipError_t e = strassen_sums(wmbej_quad_desc(c, 0.0), nullptr, c->strassen_doovv_t, c->stream);
+  if (e != hipSuccess) {
+    c->last_error = hipGetErrorString(e);
+    return CCSD_ERR_HIP;
+  }
   return CCSD_OK;
 }
 
@@ -460,8 +484,20 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
     TRY(lowprec_wmbej(c, prec));
   } else {
     // + 1/2 t_jn^bf <mn||ef>: one-level Strassen into X (free here: S is
-    // consumed), then WT += X; the classical product accumulates directly
-    TRY(square_product(c, G64(c->Tring, ov, 1, c->Doovv, ov, 1, c->WT, ov, 1, ov, ov, ov, 0.5, 1.0), c->X));
+    // consumed), then WT += X; the classical product accumulates directly.
+    // Tring's operand sums serve this product and the ring's; Doovv's are
+    // formed with Doovv.
+    StrassenSums sw;
+    if (c->strassen_tring_s) {
+      hipError_t e = strassen_sums(wmbej_quad_desc(c, 0.0), c->strassen_tring_s, nullptr, s);
+      if (e != hipSuccess) {
+        c->last_error = hipGetErrorString(e);
+        return CCSD_ERR_HIP;
+      }
+      sw.S = c->strassen_tring_s;
+      sw.T = c->strassen_doovv_t;
+    }
+    TRY(square_product(c, wmbej_quad_desc(c, 1.0), c->X, nullptr, &sw));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[4], s));
 
@@ -508,7 +544,11 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int prec, bool time_
   if (lowp) {
     TRY(lowprec_ring(c, prec));
   } else {
-    TRY(square_product(c, G64(c->Tring, ov, 1, c->WT, 1, ov, c->X, ov, 1, ov, ov, ov, 1.0, 0.0), nullptr));
+    // A = Tring with the W_mbej product's strides: its operand sums, formed there
+    StrassenSums sr;
+    sr.S = c->strassen_tring_s;
+    TRY(square_product(c, G64(c->Tring, ov, 1, c->WT, 1, ov, c->X, ov, 1, ov, ov, ov, 1.0, 0.0), nullptr, nullptr,
+                       &sr));
   }
   if (time_it) HIPTRY(hipEventRecord(c->ev[8], s));
 

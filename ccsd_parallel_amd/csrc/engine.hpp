@@ -7,6 +7,7 @@
 #include "../../include/ccsd_amd.h"
 #include "gemm.hpp"
 #include "kernels.hpp"
+#include "strassen.hpp"
 
 namespace ccsd {
 
@@ -43,8 +44,11 @@ struct Ctx {
   double *Fme = nullptr, *Fmi = nullptr, *Fae = nullptr, *Fae_t1 = nullptr;
   double *W1 = nullptr, *Q = nullptr, *W3 = nullptr, *W3u = nullptr, *W2 = nullptr, *U = nullptr;
   double *WT = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr, *Rp = nullptr, *R1 = nullptr;
-  // temporaries of the one-level Strassen (OV)^3 products (FP64), or null
+  // temporaries of the one-level Strassen (OV)^3 products (FP64), or null;
+  // with them the operand sums S1..S4 of Tring (formed once per iteration for
+  // the W_mbej and ring products) and T1..T4 of Doovv (formed with Doovv)
   double* strassen_tmp = nullptr;
+  double *strassen_tring_s = nullptr, *strassen_doovv_t = nullptr;
 
   // DIIS
   int diis_m = 0, diis_count = 0, diis_next = 0;
@@ -82,7 +86,11 @@ int compute_rhs(Ctx* c, const double* t1, const double* t2, int precision, bool 
 // C = alpha A B + beta C for the square (OV)^3 FP64 products (beta 0 or 1):
 // one level of Strassen when the context holds its temporaries and the
 // product qualifies (with beta 1 through `scratch`, M x N), else the engine.
-int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws = nullptr);
+// sums: operand sums formed beforehand for g's A / B views (strassen.hpp).
+int square_product(Ctx* c, const GemmDesc<double>& g, double* scratch, const GemmWorkspace* ws = nullptr,
+                   const StrassenSums* sums = nullptr);
+// T1..T4 of Doovv as the W_mbej product's B operand, after Doovv is (re)built.
+int strassen_doovv_sums(Ctx* c);
 int device_energy(Ctx* c, const double* t1, const double* t2, double* e);
 // strassen_min_order: as ccsd_options (0 = the default order)
 size_t plan_bytes(int O, int V, int precision, int diis_m, int strassen_min_order = 0);

@@ -102,9 +102,11 @@ bool applies(const GemmDesc<T>& d, int min_n) {
 
 // C = alpha A B on operands of type E; the result (type R) is d.C (R = E) or
 // d.Cd (R = double, E = float).  tmp: S and T (h^2 of E each), then the
-// product temporary (h^2 of R).
+// product temporary (h^2 of R).  preS / preT: the four operand sums of A / B
+// formed beforehand (strassen_sums), or null to form them here.
 template <typename E, typename R>
-hipError_t strassen_run(const GemmDesc<E>& d, R* Cr, void* tmp, const GemmWorkspace& ws, hipStream_t s) {
+hipError_t strassen_run(const GemmDesc<E>& d, R* Cr, void* tmp, const GemmWorkspace& ws, hipStream_t s,
+                        const E* preS = nullptr, const E* preT = nullptr) {
   const long h = d.M / 2;
   // Operand blocks are handled in the orientation that keeps their unit stride
   // along the columns of a view: A as [m][k] (k-contiguous) or [k][m]; B as
@@ -124,6 +126,9 @@ hipError_t strassen_run(const GemmDesc<E>& d, R* Cr, void* tmp, const GemmWorksp
   R* Mp = reinterpret_cast<R*>(Sp + 2 * h * h);
   const View<E> S{Sp, h}, T{Tp, h};
   const View<R> Tm{Mp, h};
+  // operand sum q (0-3: S1..S4 / T1..T4): the precomputed block, or the temporary
+  auto sv = [&](int q) { return preS ? View<E>{preS + q * h * h, h} : S; };
+  auto tv = [&](int q) { return preT ? View<E>{preT + q * h * h, h} : T; };
   // C(view) = alpha a b + beta C, a an A-shaped view, b a B-shaped view
   auto mul = [&](View<R> c, View<E> a, View<E> b, E alpha, E beta) {
     GemmDesc<E> g;
@@ -146,23 +151,23 @@ hipError_t strassen_run(const GemmDesc<E>& d, R* Cr, void* tmp, const GemmWorksp
   hipError_t e;
 #define ST(call) \
   if ((e = (call)) != hipSuccess) return e
-  ST(axpby(S, A21, one, A22, one, h, s));    // S1
-  ST(axpby(T, B12, one, B11, -one, h, s));   // T1
-  ST(mul(Tm, S, T, al, zero));               // M5
-  ST(axpby(S, S, one, A11, -one, h, s));     // S2
-  ST(axpby(T, B22, one, T, -one, h, s));     // T2
-  ST(mul(C12, S, T, al, zero));              // M6
-  ST(mul(C11, A11, B11, al, zero));          // M1
-  ST(sum2(C12, C21, C12, C11, h, s));        // C12 = C21 = U2 = M1 + M6
-  ST(mul(C11, A12, B21, al, one));           // C11 = M1 + M2
-  ST(axpby(S, A12, one, S, -one, h, s));     // S4
-  ST(axpby(T, T, one, B21, -one, h, s));     // T4
-  ST(sum2(C12, C22, C12, Tm, h, s));         // C12 = C22 = U4 = U2 + M5
-  ST(mul(C12, S, B22, al, one));             // C12 = U4 + M3
-  ST(mul(C21, A22, T, -al, one));            // C21 = U2 - M4
-  ST(axpby(S, A11, one, A21, -one, h, s));   // S3
-  ST(axpby(T, B22, one, B12, -one, h, s));   // T3
-  ST(mul(Tm, S, T, al, zero));               // M7
+  if (!preS) ST(axpby(S, A21, one, A22, one, h, s));   // S1
+  if (!preT) ST(axpby(T, B12, one, B11, -one, h, s));  // T1
+  ST(mul(Tm, sv(0), tv(0), al, zero));                 // M5
+  if (!preS) ST(axpby(S, S, one, A11, -one, h, s));    // S2
+  if (!preT) ST(axpby(T, B22, one, T, -one, h, s));    // T2
+  ST(mul(C12, sv(1), tv(1), al, zero));                // M6
+  ST(mul(C11, A11, B11, al, zero));                    // M1
+  ST(sum2(C12, C21, C12, C11, h, s));                  // C12 = C21 = U2 = M1 + M6
+  ST(mul(C11, A12, B21, al, one));                     // C11 = M1 + M2
+  if (!preS) ST(axpby(S, A12, one, S, -one, h, s));    // S4
+  if (!preT) ST(axpby(T, T, one, B21, -one, h, s));    // T4
+  ST(sum2(C12, C22, C12, Tm, h, s));                   // C12 = C22 = U4 = U2 + M5
+  ST(mul(C12, sv(3), B22, al, one));                   // C12 = U4 + M3
+  ST(mul(C21, A22, tv(3), -al, one));                  // C21 = U2 - M4
+  if (!preS) ST(axpby(S, A11, one, A21, -one, h, s));  // S3
+  if (!preT) ST(axpby(T, B22, one, B12, -one, h, s));  // T3
+  ST(mul(Tm, sv(2), tv(2), al, zero));                 // M7
   ST(sum2(C21, C22, Tm, View<R>{nullptr, 0}, h, s));  // C21 = U2 - M4 + M7, C22 = U4 + M7
 #undef ST
   return hipSuccess;
@@ -178,9 +183,37 @@ bool strassen_applies(const GemmDesc<float>& d, int min_n) {
 
 size_t strassen_temp_elems(long n) { return 3ull * (size_t)(n / 2) * (size_t)(n / 2); }
 
-hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s) {
+hipError_t gemm_strassen(const GemmDesc<double>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s,
+                         const StrassenSums* sums) {
   if (!strassen_applies(d, 2) || !tmp) return hipErrorInvalidValue;
-  return strassen_run<double, double>(d, d.C, tmp, ws, s);
+  return strassen_run<double, double>(d, d.C, tmp, ws, s, sums ? sums->S : nullptr, sums ? sums->T : nullptr);
+}
+
+hipError_t strassen_sums(const GemmDesc<double>& d, double* S, double* T, hipStream_t s) {
+  if (!strassen_applies(d, 2)) return hipErrorInvalidValue;
+  const long h = d.M / 2, hh = h * h;
+  const bool ak = d.sAk == 1, bk = d.sBk == 1;
+  const long lda = ak ? d.sAm : d.sAk, ldb = bk ? d.sBn : d.sBk;
+  auto ablk = [&](int bi, int bj) { return View<double>{d.A + bi * h * d.sAm + bj * h * d.sAk, lda}; };
+  auto bblk = [&](int bi, int bj) { return View<double>{d.B + bi * h * d.sBk + bj * h * d.sBn, ldb}; };
+  auto blk = [&](double* p, int q) { return View<double>{p + q * hh, h}; };
+  hipError_t e;
+#define ST(call) \
+  if ((e = (call)) != hipSuccess) return e
+  if (S) {
+    ST(axpby(blk(S, 0), ablk(1, 0), 1.0, ablk(1, 1), 1.0, h, s));    // S1 = A21 + A22
+    ST(axpby(blk(S, 1), blk(S, 0), 1.0, ablk(0, 0), -1.0, h, s));   // S2 = S1 - A11
+    ST(axpby(blk(S, 2), ablk(0, 0), 1.0, ablk(1, 0), -1.0, h, s));  // S3 = A11 - A21
+    ST(axpby(blk(S, 3), ablk(0, 1), 1.0, blk(S, 1), -1.0, h, s));   // S4 = A12 - S2
+  }
+  if (T) {
+    ST(axpby(blk(T, 0), bblk(0, 1), 1.0, bblk(0, 0), -1.0, h, s));  // T1 = B12 - B11
+    ST(axpby(blk(T, 1), bblk(1, 1), 1.0, blk(T, 0), -1.0, h, s));   // T2 = B22 - T1
+    ST(axpby(blk(T, 2), bblk(1, 1), 1.0, bblk(0, 1), -1.0, h, s));  // T3 = B22 - B12
+    ST(axpby(blk(T, 3), blk(T, 1), 1.0, bblk(1, 0), -1.0, h, s));   // T4 = T2 - B21
+  }
+#undef ST
+  return hipSuccess;
 }
 
 hipError_t gemm_strassen(const GemmDesc<float>& d, double* tmp, const GemmWorkspace& ws, hipStream_t s) {

@@ -235,6 +235,28 @@ def test_strassen_option_takes_effect():
     assert 0.0 < diff <= 1e-13 * np.abs(res[-1]).max(), diff
 
 
+def test_strassen_sums_follow_integral_updates():
+    """The operand sums of Doovv (formed once with Doovv, not per iteration)
+    are formed again when ccsd_update_integrals changes <mn||ef>: after a dense
+    update of the whole block the Strassen residual equals the oracle's for the
+    updated integrals."""
+    capi = _ctx()
+    O, V, seed = 6, 10, 24
+    prob = synth.make_problem(O, V, seed, scale=0.05)
+    rng = np.random.default_rng(seed)
+    delta = 0.01 * rng.standard_normal(prob["oovv"].shape)
+    t1, t2 = random_amplitudes(O, V, seed)
+    upd = dict(prob, oovv=prob["oovv"] + delta)
+    R1, R2 = ccsd_ref.rhs(ccsd_ref.Ints.from_problem(upd), t1, t2)[:2]
+    with capi.Context.from_problem(prob, diis_dim=0, strassen_min_order=2) as ctx:
+        ctx.residual(t1, packed(t2, O, V))  # an iteration's worth of use before the update
+        ctx.update_integrals("oovv", np.arange(delta.size), delta.ravel())
+        r1, r2 = ctx.residual(t1, packed(t2, O, V))
+    scale = max(np.abs(R2).max(), np.abs(R1).max())
+    assert np.max(np.abs(r2 - packed(R2, O, V))) <= 1e-12 * scale
+    assert np.max(np.abs(r1 - R1)) <= 1e-12 * scale
+
+
 def _check_residual(O, V, seed, **opts):
     capi = _ctx()
     prob = synth.make_problem(O, V, seed, scale=0.05)
