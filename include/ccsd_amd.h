@@ -88,8 +88,9 @@ typedef struct ccsd_options {
                                is taken at) */
   int strassen_min_order;   /* FP64 and FP32 arithmetic: the square (OV)^3 products (the ring, W_mbej's
                                quadratic term and their FP64 reverse products in ccsd_backward) use one level of
-                               Strassen-Winograd (seven half-order products, three (OV/2)^2 temporaries
-                               counted in the plan) when OV is even and at least this; 0 (default) =
+                               Strassen-Winograd (seven half-order products; three (OV/2)^2 temporaries
+                               and eight (OV/2)^2 operand sums counted in the plan) when OV is even and
+                               at least this; 0 (default) =
                                8192, < 0 = never (classical products throughout) */
 } ccsd_options;
 
