@@ -13,7 +13,8 @@
 //   C22 = M1 + M6 + M7 + M5
 // (V. Strassen, Numer. Math. 13, 354 (1969); S. Winograd's variant with 15
 // additions).  The schedule below accumulates in the C blocks (12 block
-// passes, three of them fused pairs) and needs three
+// passes, three of them fused pairs; 8 of them form the operand sums, which a
+// caller may form once with strassen_sums and pass in) and needs three
 // h x h temporaries (S and T of the operand type, one product of the result
 // type).  Each product is one engine call at
 // the half size: at n = 12000 the seven 6000^3 products and the block passes
